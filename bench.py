@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Plain SCAN throughput on one MI355X: similarity-edges/s on R-MAT-24.
+
+Workload (default): R-MAT scale 24, edge factor 16, a/b/c/d =
+.57/.19/.19/.05, seed 1, symmetrised, de-duplicated, self loops dropped,
+resident in HBM.  One step = one complete plain SCAN pass (prep, similarity,
+core check, union-find, borders, hubs/outliers) from the device-resident CSR
+to device-resident labels; the K timed steps cycle through
+eps in {0.2, 0.4, 0.6, 0.8} x mu in {2, 5, 10}.
+
+value = E * K / (wall time of the K steps) in GTEPS (1e9 similarity-edges/s).
+roofline: the similarity kernel, algorithmic bytes
+sum over undecided edges of 4 * (s + min(l, s * ceil(log2(l + 1)))) per pass,
+divided by its hipEvent-timed duration on the library's stream.
+cpu_baseline: the oracle's per-edge similarity (prep + merge intersection,
+OpenMP) over a uniform random sample of the same graph's edges, on this
+host's cores.
+
+Progress goes to stderr; stdout carries one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "similarity-edges/s (GTEPS) on R-MAT-24; served queries/s + p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, eps_list, seconds: float):
+    """Oracle similarity over a uniform sample of edges (one directed entry per
+    sampled undirected edge), calibrated to about `seconds` of CPU work."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    rp, adj = g.row_ptr, g.adj
+    m = len(adj)
+    rng = np.random.default_rng(12345)
+    per_eps = seconds / len(eps_list)
+    total_edges = 0
+    total_t = 0.0
+    for eps in eps_list:
+        probe = rng.integers(0, m, size=min(m, 20000), dtype=np.uint64)
+        t0 = time.perf_counter()
+        ob.orc_similarity_entries(rp, adj, eps, probe, threads)
+        dt = max(time.perf_counter() - t0, 1e-6)
+        count = int(min(m, max(20000, len(probe) * per_eps / dt)))
+        sample = rng.integers(0, m, size=count, dtype=np.uint64)
+        t0 = time.perf_counter()
+        ob.orc_similarity_entries(rp, adj, eps, sample, threads)
+        dt = time.perf_counter() - t0
+        total_edges += count
+        total_t += dt
+        log(f"cpu baseline eps={eps}: {count} sampled edges in {dt:.2f} s ({count / dt / 1e6:.2f} M edges/s)")
+    return {
+        "value": total_edges / total_t / 1e9,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle prep + merge-intersection similarity of {total_edges} uniformly sampled edges of the "
+                   f"same graph over eps in {eps_list} ({total_t:.1f} s, {threads} OpenMP threads); core/"
+                   f"union-find/border phases, which are O(m) passes, not included"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=float, default=16.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--eps", default="0.2,0.4,0.6,0.8")
+    ap.add_argument("--mu", default="2,5,10")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget; 0 skips it")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
+              "the SCAN path here is single-GPU and multi-GPU sharding is not built.", file=sys.stderr)
+        return 2
+
+    import efficientstructuralgraphclusteringongpus_amd as sgc
+
+    eps_list = [float(x) for x in args.eps.split(",")]
+    mu_list = [int(x) for x in args.mu.split(",")]
+    params = list(itertools.product(eps_list, mu_list))
+
+    t0 = time.perf_counter()
+    g = sgc.HostGraph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+    log(f"R-MAT-{args.scale} ef{args.edge_factor:g}: n={g.n} m={g.m} E={g.m // 2} max_deg={g.max_degree} "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    work = {eps: g.similarity_work(eps) for eps in eps_list}
+
+    ctx = sgc.Context(0)
+    t0 = time.perf_counter()
+    dg = sgc.DeviceGraph(ctx, g)
+    log(f"device {ctx.device_name}: upload {time.perf_counter() - t0:.2f} s")
+    E = g.m // 2
+
+    def step(i):
+        eps, mu = params[i % len(params)]
+        return eps, mu, sgc.scan_device(ctx, dg, eps, mu)
+
+    for i in range(args.warmup):
+        eps, mu, st = step(i)
+        log(f"warmup {i}: eps={eps} mu={mu} pass {st.ms_total:.2f} ms (similarity {st.ms_similarity:.2f} ms)")
+    ctx.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(step(i))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    for eps, mu, st in stats:
+        log(f"eps={eps} mu={mu}: pass {st.ms_total:.2f} ms, similarity {st.ms_similarity:.2f} ms, "
+            f"U={st.u_edges} cores={st.cores} clusters={st.clusters}")
+
+    ms_per_step = wall / args.steps * 1e3
+    value = E * args.steps / wall / 1e9
+    sim_ms = sum(st.ms_similarity for _, _, st in stats)
+    alg_bytes = sum(work[eps][1] for eps, _, _ in stats)
+    achieved = alg_bytes / (sim_ms / 1e3) / 1e9 if sim_ms > 0 else 0.0
+    phases = {k: sum(getattr(st, "ms_" + k) for _, _, st in stats) / len(stats)
+              for k in ("prep", "similarity", "core", "cluster", "border", "classify", "total")}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(g, eps_list, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"plain SCAN, R-MAT scale {args.scale} edge factor {args.edge_factor:g} "
+                         f"(a/b/c/d=.57/.19/.19/.05, seed {args.seed}), one pass per step over "
+                         f"eps x mu = {eps_list} x {mu_list}"),
+            "n": g.n,
+            "edges": E,
+            "max_degree": g.max_degree,
+            "device": ctx.device_name,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "similarity (k_similarity_wave)",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_launch": alg_bytes / len(stats),
+            "avg_launch_ms": sim_ms / len(stats),
+        },
+        "cpu_baseline": cpu,
+        "phases_ms_avg": {k: round(v, 3) for k, v in phases.items()},
+    }
+    print(json.dumps(out), flush=True)
+    dg.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

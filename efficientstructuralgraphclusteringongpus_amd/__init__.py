@@ -1,0 +1,382 @@
+"""MI355X-native SCAN structural graph clustering (GPUSCAN++ drop-in).
+
+Python host mirror over the C ABI of ``lib/libsgc.so`` (declared in
+``include/sgc.h``).  The compute path is the HIP library; this module only
+binds it with ctypes and moves numpy arrays.  There is no CPU fallback: if
+the library is missing or cannot be loaded every call raises.
+
+The reference's surfaces mirrored here:
+
+* ``GPUScan(folder, eps, miu)``: the ``GPUScan folder_path epsilon miu`` run
+  (GPUSCAN++/main.cpp:20-123) returning the per-vertex labels that
+  ``int* GPUScan(graph_t*, ...)`` (GPUSCAN++/GPUScan.h:7) returns.
+* ``HostGraph.load(dir)``: ``Graph::Graph(dir)`` (GPUSCAN++/Graph.cpp:147-153).
+* ``HostGraph.orient()``: ``getEidAndEdgeList`` (GPUSCAN++/Graph.cpp:52-141).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+__all__ = [
+    "SGCError", "HostGraph", "Context", "DeviceGraph", "ScanStats", "ScanResult",
+    "scan", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
+]
+
+HUB = -1
+OUTLIER = -2
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def library_path() -> str:
+    return os.path.join(_PKG, "lib", "libsgc.so")
+
+
+class SGCError(RuntimeError):
+    """A libsgc call failed; ``code`` is the SGC_E* status."""
+
+    CODES = {1: "EINVAL", 2: "EIO", 3: "EFORMAT", 4: "ENOMEM", 5: "EDEVICE", 6: "ERANGE"}
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{self.CODES.get(code, code)}] {message}")
+        self.code = code
+        self.message = message
+
+
+class _Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double), ("ms_prep", C.c_double), ("ms_similarity", C.c_double),
+        ("ms_core", C.c_double), ("ms_cluster", C.c_double), ("ms_border", C.c_double),
+        ("ms_classify", C.c_double),
+        ("edges", C.c_uint64), ("u_edges", C.c_uint64), ("similar_edges", C.c_uint64),
+        ("cores", C.c_uint64), ("clusters", C.c_uint64), ("borders", C.c_uint64),
+        ("hubs", C.c_uint64), ("outliers", C.c_uint64),
+        ("alg_bytes", C.c_double), ("probe_elements", C.c_double),
+    ]
+
+
+class _GraphT(C.Structure):
+    _fields_ = [("n", C.c_long), ("m", C.c_long), ("adj", C.POINTER(C.c_int32)),
+                ("num_edges", C.POINTER(C.c_uint32)), ("eid", C.POINTER(C.c_uint32))]
+
+
+@dataclass
+class ScanStats:
+    ms_total: float
+    ms_prep: float
+    ms_similarity: float
+    ms_core: float
+    ms_cluster: float
+    ms_border: float
+    ms_classify: float
+    edges: int
+    u_edges: int
+    similar_edges: int
+    cores: int
+    clusters: int
+    borders: int
+    hubs: int
+    outliers: int
+    alg_bytes: float
+    probe_elements: float
+
+    @classmethod
+    def _from(cls, s: _Stats) -> "ScanStats":
+        return cls(**{f.name: getattr(s, f.name) for f in fields(cls)})
+
+
+@dataclass
+class ScanResult:
+    labels: np.ndarray          # int32[n]: cluster id (min core id), -1 hub, -2 outlier
+    core: np.ndarray            # uint8[n]
+    sim: np.ndarray | None      # uint8[m] per directed CSR entry (1 = similar)
+    stats: ScanStats
+
+
+_VP = C.c_void_p
+_P = C.POINTER
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def load_library(path: str | None = None):
+    """Loads libsgc.so once; raises if it is missing (no fallback path)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = path or library_path()
+    if not os.path.exists(path):
+        raise SGCError(5, f"{path} is not built; run efficientstructuralgraphclusteringongpus_amd.build")
+    lib = C.CDLL(path)
+    _sig(lib, "sgc_last_error", C.c_char_p, [])
+    _sig(lib, "sgc_version", C.c_char_p, [])
+    _sig(lib, "sgc_graph_load", C.c_int, [C.c_char_p, C.c_int, _P(_VP)])
+    _sig(lib, "sgc_graph_save", C.c_int, [_VP, C.c_char_p])
+    _sig(lib, "sgc_graph_from_csr", C.c_int, [C.c_uint64, C.c_uint64, _VP, _VP, C.c_int, _P(_VP)])
+    _sig(lib, "sgc_graph_from_edges", C.c_int, [C.c_uint64, C.c_uint64, _VP, _VP, _P(_VP)])
+    _sig(lib, "sgc_graph_convert_edge_list", C.c_int, [C.c_char_p, C.c_char_p])
+    _sig(lib, "sgc_graph_rmat", C.c_int,
+         [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, _P(_VP)])
+    _sig(lib, "sgc_graph_planted", C.c_int,
+         [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_uint32,
+          C.c_double, C.c_uint64, _P(_VP)])
+    _sig(lib, "sgc_graph_n", C.c_uint64, [_VP])
+    _sig(lib, "sgc_graph_m", C.c_uint64, [_VP])
+    _sig(lib, "sgc_graph_row_ptr", _P(C.c_uint64), [_VP])
+    _sig(lib, "sgc_graph_adj", _P(C.c_int32), [_VP])
+    _sig(lib, "sgc_graph_max_degree", C.c_uint32, [_VP])
+    _sig(lib, "sgc_graph_digest", C.c_uint64, [_VP])
+    _sig(lib, "sgc_graph_free", None, [_VP])
+    _sig(lib, "sgc_graph_similarity_work", C.c_int,
+         [_VP, C.c_double, _P(C.c_uint64), _P(C.c_double), _P(C.c_double)])
+    _sig(lib, "sgc_philox4x32_10", None, [_VP, _VP, _VP])
+    _sig(lib, "sgc_orient", C.c_int, [_VP, _VP, _VP, _VP])
+    _sig(lib, "sgc_ctx_create", C.c_int, [C.c_int, _P(_VP)])
+    _sig(lib, "sgc_ctx_destroy", None, [_VP])
+    _sig(lib, "sgc_ctx_sync", C.c_int, [_VP])
+    _sig(lib, "sgc_ctx_device_name", C.c_char_p, [_VP])
+    _sig(lib, "sgc_dgraph_upload", C.c_int, [_VP, _VP, _P(_VP)])
+    _sig(lib, "sgc_dgraph_free", None, [_VP])
+    _sig(lib, "sgc_dgraph_n", C.c_uint64, [_VP])
+    _sig(lib, "sgc_dgraph_m", C.c_uint64, [_VP])
+    _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
+    _sig(lib, "sgc_scan_device", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
+    _sig(lib, "sgc_GPUScan", _P(C.c_int32), [_P(_GraphT), _VP, _VP, C.c_double, C.c_int32])
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SGCError(rc, _LIB.sgc_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_VP)
+
+
+class HostGraph:
+    """Host CSR graph owned by libsgc (``sgc_host_graph``)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = load_library()
+
+    def __del__(self):
+        self.close()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgc_graph_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def load(cls, folder: str, check_symmetry: bool = False) -> "HostGraph":
+        lib = load_library()
+        h = _VP()
+        _check(lib.sgc_graph_load(os.fsencode(folder), int(check_symmetry), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, row_ptr, adj, check_symmetry: bool = True) -> "HostGraph":
+        lib = load_library()
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        ad = np.ascontiguousarray(adj, dtype=np.int32)
+        h = _VP()
+        _check(lib.sgc_graph_from_csr(len(rp) - 1, len(ad), _ptr(rp), _ptr(ad), int(check_symmetry), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n: int, src, dst) -> "HostGraph":
+        lib = load_library()
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        d = np.ascontiguousarray(dst, dtype=np.int32)
+        h = _VP()
+        _check(lib.sgc_graph_from_edges(n, len(s), _ptr(s), _ptr(d), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: float = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> "HostGraph":
+        lib = load_library()
+        h = _VP()
+        _check(lib.sgc_graph_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def planted(cls, n: int = 2_000_000, communities: int = 20_000, zipf: float = 1.5, p_in: float = 0.05,
+                max_intra_degree: float = 200.0, inter_per_vertex: int = 8, hubs: int = 64,
+                hub_fraction: float = 0.05, seed: int = 1) -> "HostGraph":
+        lib = load_library()
+        h = _VP()
+        _check(lib.sgc_graph_planted(n, communities, zipf, p_in, max_intra_degree, inter_per_vertex, hubs,
+                                     hub_fraction, seed, C.byref(h)))
+        return cls(h)
+
+    @staticmethod
+    def convert_edge_list(txt_path: str, out_dir: str) -> None:
+        lib = load_library()
+        _check(lib.sgc_graph_convert_edge_list(os.fsencode(txt_path), os.fsencode(out_dir)))
+
+    def save(self, folder: str) -> None:
+        os.makedirs(folder, exist_ok=True)
+        _check(self._lib.sgc_graph_save(self._h, os.fsencode(folder)))
+
+    @property
+    def n(self) -> int:
+        return int(self._lib.sgc_graph_n(self._h))
+
+    @property
+    def m(self) -> int:
+        return int(self._lib.sgc_graph_m(self._h))
+
+    @property
+    def row_ptr(self) -> np.ndarray:
+        """Read-only view (valid while the graph is alive)."""
+        p = self._lib.sgc_graph_row_ptr(self._h)
+        a = np.ctypeslib.as_array(p, shape=(self.n + 1,))
+        a.flags.writeable = False
+        return a
+
+    @property
+    def adj(self) -> np.ndarray:
+        m = self.m
+        if m == 0:
+            return np.zeros(0, dtype=np.int32)
+        a = np.ctypeslib.as_array(self._lib.sgc_graph_adj(self._h), shape=(m,))
+        a.flags.writeable = False
+        return a
+
+    @property
+    def max_degree(self) -> int:
+        return int(self._lib.sgc_graph_max_degree(self._h))
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row_ptr).astype(np.int64)
+
+    def digest(self) -> int:
+        return int(self._lib.sgc_graph_digest(self._h))
+
+    def similarity_work(self, eps: float):
+        """(undecided edges, algorithmic bytes, short-row elements) at eps."""
+        u = C.c_uint64()
+        b = C.c_double()
+        sh = C.c_double()
+        _check(self._lib.sgc_graph_similarity_work(self._h, float(eps), C.byref(u), C.byref(b), C.byref(sh)))
+        return int(u.value), float(b.value), float(sh.value)
+
+    def orient(self):
+        """(edge_u, edge_v, eid) as getEidAndEdgeList builds them."""
+        E = self.m // 2
+        eu = np.empty(max(E, 1), dtype=np.int32)
+        ev = np.empty(max(E, 1), dtype=np.int32)
+        eid = np.empty(max(self.m, 1), dtype=np.uint32)
+        _check(self._lib.sgc_orient(self._h, _ptr(eu), _ptr(ev), _ptr(eid)))
+        return eu[:E], ev[:E], eid[: self.m]
+
+
+class Context:
+    """A HIP device and stream (``sgc_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = _VP()
+        _check(self._lib.sgc_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device_name(self) -> str:
+        return self._lib.sgc_ctx_device_name(self._h).decode()
+
+    def sync(self) -> None:
+        _check(self._lib.sgc_ctx_sync(self._h))
+
+
+class DeviceGraph:
+    """A CSR graph resident in HBM (``sgc_dgraph``)."""
+
+    def __init__(self, ctx: Context, graph: HostGraph):
+        self._lib = load_library()
+        self.ctx = ctx
+        h = _VP()
+        _check(self._lib.sgc_dgraph_upload(ctx.handle, graph.handle, C.byref(h)))
+        self._h = h
+        self.n = graph.n
+        self.m = graph.m
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgc_dgraph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def scan(ctx: Context, dg: DeviceGraph, eps: float, mu: int, want_sim: bool = False) -> ScanResult:
+    """One plain SCAN pass on a resident graph; returns host copies."""
+    lib = load_library()
+    labels = np.empty(max(dg.n, 1), dtype=np.int32)
+    core = np.empty(max(dg.n, 1), dtype=np.uint8)
+    sim = np.empty(max(dg.m, 1), dtype=np.uint8) if want_sim else None
+    st = _Stats()
+    _check(lib.sgc_scan(ctx.handle, dg.handle, float(eps), int(mu), _ptr(labels), _ptr(core), _ptr(sim),
+                        C.byref(st)))
+    return ScanResult(labels[: dg.n], core[: dg.n], None if sim is None else sim[: dg.m], ScanStats._from(st))
+
+
+def scan_device(ctx: Context, dg: DeviceGraph, eps: float, mu: int, d_labels: int | None = None,
+                d_core: int | None = None, d_sim: int | None = None) -> ScanStats:
+    """One pass leaving results in HBM (optionally copied to caller device
+    buffers given as raw pointers); returns the pass statistics."""
+    lib = load_library()
+    st = _Stats()
+    _check(lib.sgc_scan_device(ctx.handle, dg.handle, float(eps), int(mu), d_labels, d_core, d_sim,
+                               C.byref(st)))
+    return ScanStats._from(st)
+
+
+def GPUScan(folder: str, eps: float, miu: int) -> np.ndarray:
+    """The reference's run on ``folder`` (b_degree.bin/b_adj.bin): labels[n]."""
+    g = HostGraph.load(folder)
+    with Context(0) as ctx:
+        dg = DeviceGraph(ctx, g)
+        try:
+            return scan(ctx, dg, eps, miu).labels
+        finally:
+            dg.close()

@@ -1,0 +1,96 @@
+// Device-side structures shared by the C-ABI implementation and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+#define SGC_HIP(call)                                                                             \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return ::sgc::fail(SGC_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                               \
+  } while (0)
+
+namespace sgc {
+
+constexpr int kWave = 64;
+constexpr int32_t kNoLabel = 0x7FFFFFFF;
+
+// Per-edge flag byte stored for every directed CSR entry.
+enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagU = 2 };
+
+// Counters written by the kernels (one 64-bit slot each).
+enum Counter {
+  kCntUEdges = 0,
+  kCntSimilar,
+  kCntCores,
+  kCntClusters,
+  kCntBorders,
+  kCntHubs,
+  kCntOutliers,
+  kCntProbes,
+  kCntSlots = 16
+};
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace sgc
+
+struct sgc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char name[256] = {0};
+  int cu_count = 0;
+  sgc::DevBuf work;      // per-pass arrays
+  sgc::DevBuf counters;  // kCntSlots x uint64
+  uint64_t* host_counters = nullptr;  // pinned mirror
+  hipEvent_t ev[8] = {};
+};
+
+struct sgc_dgraph {
+  sgc_ctx* ctx = nullptr;
+  uint32_t n = 0;
+  uint32_t m = 0;
+  uint32_t max_degree = 0;
+  uint32_t* row_ptr = nullptr;  // n + 1
+  int32_t* adj = nullptr;       // m
+  uint32_t* src = nullptr;      // m: row of each entry
+};
+
+namespace sgc {
+
+// Arrays one SCAN pass needs on the device.
+struct PassBuffers {
+  uint8_t* flag;     // m
+  uint32_t* sd;      // n: similar-degree counts
+  int32_t* label;    // n: union-find parent, then cluster id
+  int32_t* lo;       // n: classify min neighbour label
+  int32_t* hi;       // n: classify max neighbour label
+  uint8_t* core;     // n
+  uint32_t* ulist;   // m/2: undecided edges, by entry index in the higher row
+  uint64_t* cnt;     // kCntSlots
+};
+
+struct ScanParams {
+  double eps;
+  int32_t mu;
+};
+
+// Kernel launchers (kernels.hip).  All run on `s` and never synchronise.
+void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
+void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
+void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+
+}  // namespace sgc

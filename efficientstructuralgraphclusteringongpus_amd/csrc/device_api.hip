@@ -1,0 +1,309 @@
+// C-ABI: device contexts, device-resident graphs and the plain SCAN pass.
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "device.h"
+#include "graph_host.h"
+
+using namespace sgc;
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return SGC_OK;
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(SGC_ENOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return SGC_OK;
+}
+
+// Carves the per-pass arrays out of the context workspace.
+int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
+  const size_t n = g->n, m = g->m;
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + std::max<size_t>(bytes, 1), 256);
+    return o;
+  };
+  const size_t o_flag = take(m);
+  const size_t o_sd = take(n * 4);
+  const size_t o_label = take(n * 4);
+  const size_t o_lo = take(n * 4);
+  const size_t o_hi = take(n * 4);
+  const size_t o_core = take(n);
+  const size_t o_ulist = take(m * 4);
+  int rc = ensure(ctx->work, off);
+  if (rc) return rc;
+  char* base = (char*)ctx->work.ptr;
+  pb->flag = (uint8_t*)(base + o_flag);
+  pb->sd = (uint32_t*)(base + o_sd);
+  pb->label = (int32_t*)(base + o_label);
+  pb->lo = (int32_t*)(base + o_lo);
+  pb->hi = (int32_t*)(base + o_hi);
+  pb->core = (uint8_t*)(base + o_core);
+  pb->ulist = (uint32_t*)(base + o_ulist);
+  pb->cnt = (uint64_t*)ctx->counters.ptr;
+  return SGC_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+}  // namespace
+
+extern "C" int sgc_ctx_create(int device, sgc_ctx** out) {
+  clear_error();
+  if (!out) return fail(SGC_EINVAL, "null argument");
+  *out = nullptr;
+  int count = 0;
+  SGC_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(SGC_EINVAL, "device %d not present (%d devices)", device, count);
+  SGC_HIP(hipSetDevice(device));
+  sgc_ctx* ctx = new sgc_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  SGC_HIP(hipGetDeviceProperties(&prop, device));
+  strncpy(ctx->name, prop.name, sizeof(ctx->name) - 1);
+  ctx->cu_count = prop.multiProcessorCount;
+  SGC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  int rc = ensure(ctx->counters, kCntSlots * sizeof(uint64_t));
+  if (rc) {
+    delete ctx;
+    return rc;
+  }
+  SGC_HIP(hipHostMalloc((void**)&ctx->host_counters, kCntSlots * sizeof(uint64_t), hipHostMallocDefault));
+  for (auto& e : ctx->ev) SGC_HIP(hipEventCreate(&e));
+  *out = ctx;
+  return SGC_OK;
+}
+
+extern "C" void sgc_ctx_destroy(sgc_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->work.ptr) (void)hipFree(ctx->work.ptr);
+  if (ctx->counters.ptr) (void)hipFree(ctx->counters.ptr);
+  if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int sgc_ctx_sync(sgc_ctx* ctx) {
+  clear_error();
+  if (!ctx) return fail(SGC_EINVAL, "null context");
+  SGC_HIP(hipSetDevice(ctx->device));
+  SGC_HIP(hipStreamSynchronize(ctx->stream));
+  return SGC_OK;
+}
+
+extern "C" const char* sgc_ctx_device_name(const sgc_ctx* ctx) { return ctx ? ctx->name : ""; }
+
+extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out) {
+  clear_error();
+  if (!ctx || !g || !out) return fail(SGC_EINVAL, "null argument");
+  *out = nullptr;
+  if (g->m >= (1ull << 32) || g->n >= (1ull << 31))
+    return fail(SGC_ERANGE, "graph with n=%llu, m=%llu exceeds the in-memory path (m < 2^32); use sgc_scan_ooc",
+                (unsigned long long)g->n, (unsigned long long)g->m);
+  SGC_HIP(hipSetDevice(ctx->device));
+  sgc_dgraph* d = new sgc_dgraph();
+  d->ctx = ctx;
+  d->n = (uint32_t)g->n;
+  d->m = (uint32_t)g->m;
+  std::vector<uint32_t> rp32(g->n + 1);
+  uint64_t maxd = 0;
+  for (uint64_t u = 0; u <= g->n; ++u) {
+    rp32[u] = (uint32_t)g->row_ptr[u];
+    if (u < g->n) maxd = std::max<uint64_t>(maxd, g->row_ptr[u + 1] - g->row_ptr[u]);
+  }
+  d->max_degree = (uint32_t)maxd;
+  auto cleanup = [d]() {
+    if (d->row_ptr) (void)hipFree(d->row_ptr);
+    if (d->adj) (void)hipFree(d->adj);
+    if (d->src) (void)hipFree(d->src);
+    delete d;
+  };
+  const size_t m_bytes = std::max<size_t>(g->m, 1) * 4;
+  if (hipMalloc(&d->row_ptr, (g->n + 1) * 4) != hipSuccess || hipMalloc(&d->adj, m_bytes) != hipSuccess ||
+      hipMalloc(&d->src, m_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    cleanup();
+    return fail(SGC_ENOMEM, "device allocation for graph (n=%llu, m=%llu) failed", (unsigned long long)g->n,
+                (unsigned long long)g->m);
+  }
+  hipError_t e = hipMemcpyAsync(d->row_ptr, rp32.data(), (g->n + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && g->m)
+    e = hipMemcpyAsync(d->adj, g->adj, g->m * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch_fill_src(d, ctx->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(SGC_EDEVICE, "graph upload failed: %s", hipGetErrorString(e));
+  }
+  *out = d;
+  return SGC_OK;
+}
+
+extern "C" void sgc_dgraph_free(sgc_dgraph* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->ctx->device);
+  (void)hipStreamSynchronize(d->ctx->stream);
+  if (d->row_ptr) (void)hipFree(d->row_ptr);
+  if (d->adj) (void)hipFree(d->adj);
+  if (d->src) (void)hipFree(d->src);
+  delete d;
+}
+
+extern "C" uint64_t sgc_dgraph_n(const sgc_dgraph* d) { return d ? d->n : 0; }
+extern "C" uint64_t sgc_dgraph_m(const sgc_dgraph* d) { return d ? d->m : 0; }
+
+namespace {
+
+int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* pb, sgc_stats* st) {
+  SGC_HIP(hipSetDevice(ctx->device));
+  int rc = carve(ctx, g, pb);
+  if (rc) return rc;
+  const ScanParams p{eps, mu};
+  hipStream_t s = ctx->stream;
+  SGC_HIP(hipEventRecord(ctx->ev[0], s));
+  launch_init(g, *pb, s);
+  launch_prep(g, *pb, p, s);
+  SGC_HIP(hipEventRecord(ctx->ev[1], s));
+  launch_similarity(g, *pb, p, ctx->cu_count, s);
+  SGC_HIP(hipEventRecord(ctx->ev[2], s));
+  launch_core(g, *pb, p, s);
+  SGC_HIP(hipEventRecord(ctx->ev[3], s));
+  launch_cluster(g, *pb, s);
+  SGC_HIP(hipEventRecord(ctx->ev[4], s));
+  launch_border(g, *pb, s);
+  SGC_HIP(hipEventRecord(ctx->ev[5], s));
+  launch_classify(g, *pb, p, s);
+  SGC_HIP(hipEventRecord(ctx->ev[6], s));
+  SGC_HIP(hipGetLastError());
+  SGC_HIP(hipMemcpyAsync(ctx->host_counters, pb->cnt, kCntSlots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SGC_HIP(hipStreamSynchronize(s));
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->ms_prep = elapsed(ctx->ev[0], ctx->ev[1]);
+    st->ms_similarity = elapsed(ctx->ev[1], ctx->ev[2]);
+    st->ms_core = elapsed(ctx->ev[2], ctx->ev[3]);
+    st->ms_cluster = elapsed(ctx->ev[3], ctx->ev[4]);
+    st->ms_border = elapsed(ctx->ev[4], ctx->ev[5]);
+    st->ms_classify = elapsed(ctx->ev[5], ctx->ev[6]);
+    st->ms_total = elapsed(ctx->ev[0], ctx->ev[6]);
+    const uint64_t* c = ctx->host_counters;
+    st->edges = g->m / 2;
+    st->u_edges = c[kCntUEdges];
+    st->similar_edges = c[kCntSimilar] / 2;
+    st->cores = c[kCntCores];
+    st->clusters = c[kCntClusters];
+    st->borders = c[kCntBorders];
+    st->hubs = c[kCntHubs];
+    st->outliers = c[kCntOutliers];
+    st->probe_elements = (double)c[kCntProbes];
+  }
+  return SGC_OK;
+}
+
+}  // namespace
+
+extern "C" int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int32_t* d_labels,
+                               uint8_t* d_core, uint8_t* d_sim, sgc_stats* st) {
+  clear_error();
+  if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(SGC_EINVAL, "graph belongs to another context");
+  PassBuffers pb;
+  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  if (d_labels) SGC_HIP(hipMemcpyAsync(d_labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
+  if (d_core) SGC_HIP(hipMemcpyAsync(d_core, pb.core, g->n, hipMemcpyDeviceToDevice, s));
+  if (d_sim && g->m) SGC_HIP(hipMemcpyAsync(d_sim, pb.flag, g->m, hipMemcpyDeviceToDevice, s));
+  SGC_HIP(hipStreamSynchronize(s));
+  return SGC_OK;
+}
+
+extern "C" int sgc_scan(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int32_t* labels, uint8_t* core,
+                        uint8_t* sim, sgc_stats* st) {
+  clear_error();
+  if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(SGC_EINVAL, "graph belongs to another context");
+  PassBuffers pb;
+  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  if (labels) SGC_HIP(hipMemcpyAsync(labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
+  if (core) SGC_HIP(hipMemcpyAsync(core, pb.core, g->n, hipMemcpyDeviceToHost, s));
+  if (sim && g->m) SGC_HIP(hipMemcpyAsync(sim, pb.flag, g->m, hipMemcpyDeviceToHost, s));
+  SGC_HIP(hipStreamSynchronize(s));
+  return SGC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-compatible entry point
+// ---------------------------------------------------------------------------
+
+namespace {
+std::mutex g_compat_mu;
+sgc_ctx* g_compat_ctx = nullptr;
+}  // namespace
+
+extern "C" int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v, double eps, int32_t miu) {
+  (void)Edge_u;
+  (void)Edge_v;
+  clear_error();
+  if (!g || g->n < 0 || g->m < 0 || (g->m && (!g->adj || !g->num_edges))) {
+    fail(SGC_EINVAL, "invalid graph_t");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_compat_mu);
+  if (!g_compat_ctx && sgc_ctx_create(0, &g_compat_ctx) != SGC_OK) return nullptr;
+  sgc_host_graph hg;
+  hg.n = (uint64_t)g->n;
+  hg.m = (uint64_t)g->m;
+  std::vector<uint64_t> rp(hg.n + 1);
+  for (uint64_t u = 0; u <= hg.n; ++u) rp[u] = g->num_edges[u];
+  hg.row_ptr = rp.data();
+  hg.adj = g->adj;
+  sgc_dgraph* dg = nullptr;
+  int rc = sgc_dgraph_upload(g_compat_ctx, &hg, &dg);
+  hg.row_ptr = nullptr;
+  hg.adj = nullptr;
+  if (rc) return nullptr;
+  int32_t* out = (int32_t*)malloc(std::max<size_t>(hg.n, 1) * sizeof(int32_t));
+  sgc_stats st;
+  rc = out ? sgc_scan(g_compat_ctx, dg, eps, miu, out, nullptr, nullptr, &st) : fail(SGC_ENOMEM, "malloc failed");
+  sgc_dgraph_free(dg);
+  if (rc) {
+    free(out);
+    return nullptr;
+  }
+  fprintf(stderr, "\nPrep time: %.3lf s\n", st.ms_prep / 1000.0);
+  fprintf(stderr, "\nCheck Core time: %.3lf s\n", (st.ms_similarity + st.ms_core) / 1000.0);
+  fprintf(stderr, "core number: %llu\n", (unsigned long long)st.cores);
+  fprintf(stderr, "\nFirst Link Phase time: %.3lf s\n", st.ms_cluster / 1000.0);
+  fprintf(stderr, "Cluster non-core time: %.3lf s\n", st.ms_border / 1000.0);
+  fprintf(stderr, "\nClassify Others time: %.3lf s\n", st.ms_classify / 1000.0);
+  return out;
+}

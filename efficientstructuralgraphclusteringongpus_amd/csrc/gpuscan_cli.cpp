@@ -1,0 +1,116 @@
+// GPUScan <folder> <eps> <miu> [options]
+//
+// Drop-in for the reference command line (GPUSCAN++/main.cpp:20-123): same
+// positional arguments (eps by atof, miu by atoi, :26-27), same usage error,
+// same stdout/stderr lines for the parity-relevant outputs ("max degree",
+// "node number and edge number", "core number", "Cluster Number").
+// Options beyond the reference:
+//   --out <file>      write labels as int32[n] followed by core flags uint8[n]
+//   --device <id>     HIP device (default 0)
+//   --check-symmetry  also verify that every (u,v) has (v,u)
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "sgc.h"
+
+using namespace std::chrono;
+
+static double seconds_since(high_resolution_clock::time_point t0) {
+  return duration_cast<milliseconds>(high_resolution_clock::now() - t0).count() / 1000.0;
+}
+
+int main(int argc, char* argv[]) {
+  if (argc < 4) {
+    fprintf(stderr, "%s <Graph file> , parameter eps and miu\n", argv[0]);
+    return 1;
+  }
+  const double eps = atof(argv[2]);
+  const int miu = atoi(argv[3]);
+  const char* out_path = nullptr;
+  int device = 0;
+  int check_symmetry = 0;
+  for (int i = 4; i < argc; ++i) {
+    if (!strcmp(argv[i], "--out") && i + 1 < argc)
+      out_path = argv[++i];
+    else if (!strcmp(argv[i], "--device") && i + 1 < argc)
+      device = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--check-symmetry"))
+      check_symmetry = 1;
+    else {
+      fprintf(stderr, "unknown option %s\n", argv[i]);
+      return 1;
+    }
+  }
+
+  auto t_load = high_resolution_clock::now();
+  sgc_host_graph* g = nullptr;
+  if (sgc_graph_load(argv[1], check_symmetry, &g) != SGC_OK) {
+    fprintf(stderr, "%s\n", sgc_last_error());
+    return 1;
+  }
+  fprintf(stderr, "read graph time: %.3lf s\n", seconds_since(t_load));
+  const uint64_t n = sgc_graph_n(g), m = sgc_graph_m(g);
+  printf("max degree: %u\n", sgc_graph_max_degree(g));
+  fflush(stdout);
+
+  auto t_scan = high_resolution_clock::now();
+  fprintf(stderr, "node number and edge number: %ld, %ld\n", (long)n, (long)m);
+  fprintf(stderr, "Start GPUScan\n");
+  sgc_ctx* ctx = nullptr;
+  sgc_dgraph* dg = nullptr;
+  if (sgc_ctx_create(device, &ctx) != SGC_OK) {
+    fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+    return 1;
+  }
+  auto t_mem = high_resolution_clock::now();
+  if (sgc_dgraph_upload(ctx, g, &dg) != SGC_OK) {
+    fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+    return 1;
+  }
+  fprintf(stderr, "Create GPU Memory time: %.3lf s\n", seconds_since(t_mem));
+  std::vector<int32_t> labels(n ? n : 1);
+  std::vector<uint8_t> core(n ? n : 1);
+  sgc_stats st;
+  if (sgc_scan(ctx, dg, eps, miu, labels.data(), core.data(), nullptr, &st) != SGC_OK) {
+    fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+    return 1;
+  }
+  fprintf(stderr, "\nPrep time: %.3lf s\n", st.ms_prep / 1000.0);
+  fprintf(stderr, "\nCheck Core time: %.3lf s\n", (st.ms_similarity + st.ms_core) / 1000.0);
+  fprintf(stderr, "core number: %llu\n", (unsigned long long)st.cores);
+  fprintf(stderr, "\nFirst Link Phase time: %.3lf s\n", st.ms_cluster / 1000.0);
+  fprintf(stderr, "Cluster non-core time: %.3lf s\n", st.ms_border / 1000.0);
+  fprintf(stderr, "\nClassify Others time: %.3lf s\n", st.ms_classify / 1000.0);
+  fprintf(stderr, "device pass time: %.3lf ms (similarity %.3lf ms, %llu undecided edges)\n", st.ms_total,
+          st.ms_similarity, (unsigned long long)st.u_edges);
+  fprintf(stderr, "Finish GPUScan\n");
+  fprintf(stderr, "gpuscan time: %.3lf s\n", seconds_since(t_scan));
+
+  // Distinct labels >= 0 (GPUSCAN++/main.cpp:83-111).
+  std::vector<char> seen(n ? n : 1, 0);
+  long clusters = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (labels[i] >= 0 && !seen[labels[i]]) {
+      seen[labels[i]] = 1;
+      ++clusters;
+    }
+  fprintf(stderr, "Cluster Number: %ld\n", clusters);
+
+  if (out_path) {
+    FILE* f = fopen(out_path, "wb");
+    if (!f || fwrite(labels.data(), 4, n, f) != n || fwrite(core.data(), 1, n, f) != n) {
+      fprintf(stderr, "cannot write %s\n", out_path);
+      return 1;
+    }
+    fclose(f);
+  }
+  sgc_dgraph_free(dg);
+  sgc_ctx_destroy(ctx);
+  sgc_graph_free(g);
+  fprintf(stderr, "Work Complete!\n");
+  return 0;
+}

@@ -1,0 +1,28 @@
+// Host CSR container shared by the loader, generators, the device upload and
+// the out-of-core tier.
+#pragma once
+
+#include <stdint.h>
+
+#include "common.h"
+
+struct sgc_host_graph {
+  uint64_t n = 0;
+  uint64_t m = 0;
+  uint64_t* row_ptr = nullptr;  // n + 1 offsets
+  int32_t* adj = nullptr;       // m neighbour ids, rows strictly increasing
+  bool pinned = false;          // adj allocated with hipHostMalloc
+};
+
+namespace sgc {
+
+sgc_host_graph* new_host_graph(uint64_t n, uint64_t m, bool pinned);
+// Full contract check: ids < n, rows strictly increasing, no self loops and,
+// if requested, every (u,v) has its (v,u).
+int validate_graph(const sgc_host_graph* g, bool check_symmetry);
+// Builds a simple undirected graph from `count` directed draws: self loops
+// dropped, both directions added, duplicates removed, rows sorted.
+int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint32_t* dst,
+                     sgc_host_graph** out);
+
+}  // namespace sgc

@@ -1,0 +1,477 @@
+// SCAN kernels for gfx950 (wave64).  Every kernel is a grid-stride loop over
+// vertices or directed CSR entries; per-row aggregation inside a wave uses a
+// segmented shuffle reduction (entries of one row are consecutive lanes) so
+// each row issues at most one atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace sgc {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ uint32_t deg_of(const uint32_t* rp, uint32_t x) { return rp[x + 1] - rp[x]; }
+
+// Loads that bypass the non-coherent per-CU cache: union-find reads parents
+// that other CUs are changing.
+__device__ __forceinline__ int32_t ld_coherent(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(int32_t* p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Segmented suffix reduction over a wave: lanes with equal `row` form
+// contiguous segments; on return the first lane of each segment holds the
+// reduction of its segment.  All 64 lanes must call it.
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_seg_reduce(uint32_t row, T val, Op op) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    T ov = __shfl_down(val, off);
+    uint32_t orow = __shfl_down(row, off);
+    if (lane + off < kWave && orow == row) val = op(val, ov);
+  }
+  return val;
+}
+
+__device__ __forceinline__ bool wave_seg_head(uint32_t row) {
+  uint32_t prev = __shfl_up(row, 1);
+  return lane_id() == 0 || prev != row;
+}
+
+struct OpAdd {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+struct OpMin {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a < b ? a : b; }
+};
+struct OpMax {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
+};
+
+// ---------------------------------------------------------------------------
+// Graph preparation
+// ---------------------------------------------------------------------------
+
+// src[k] = row of entry k; one wave per vertex, lanes stride the row.
+__global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t* __restrict__ rp,
+                                                     uint32_t* __restrict__ src) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t x = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; x < n; x += waves) {
+    const uint32_t b = rp[x], e = rp[x + 1];
+    for (uint32_t k = b + lane; k < e; k += kWave) src[k] = (uint32_t)x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pass initialisation
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
+                                                 int32_t* __restrict__ hi, uint64_t* __restrict__ cnt) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    sd[x] = 0;
+    lo[x] = kNoLabel;
+    hi[x] = -1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Degree-only pre-decision (GPUSCAN++/GPUScan.cu:133-172)
+// ---------------------------------------------------------------------------
+
+// For every directed entry (u -> v): orient the edge (a = lower (degree, id)
+// endpoint, GPUSCAN++/Graph.cpp:75-83), apply the reference's two degree
+// tests in its own operand order, and store N / Y / U.  Undecided edges are
+// listed once, by their entry in the higher endpoint's row.
+__global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ adj, const uint32_t* __restrict__ src,
+                                                 double eps, uint8_t* __restrict__ flag,
+                                                 uint32_t* __restrict__ ulist, uint64_t* __restrict__ cnt) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
+    const uint64_t k = base + lane;
+    bool listed = false;
+    if (k < m) {
+      const uint32_t u = src[k];
+      const uint32_t v = (uint32_t)adj[k];
+      const uint32_t du = deg_of(rp, u), dv = deg_of(rp, v);
+      const bool up = dv > du || (dv == du && v > u);
+      const uint32_t da = up ? du : dv, db = up ? dv : du;
+      const double A = (double)(da + 1u);
+      const double B = (double)(db + 1u);
+      uint8_t f;
+      if (A < B * eps * eps)
+        f = kFlagN;
+      else if (eps * eps * A * B <= 4.0)
+        f = kFlagY;
+      else
+        f = kFlagU;
+      flag[k] = f;
+      listed = f == kFlagU && !up;
+    }
+    const uint64_t mask = __ballot(listed);
+    if (mask) {
+      const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t pos = 0;
+      if (lane == leader) pos = (uint32_t)atomicAdd((unsigned long long*)&cnt[kCntUEdges], (unsigned long long)__popcll(mask));
+      pos = __shfl(pos, leader);
+      if (listed) ulist[pos + __popcll(mask & ((1ull << lane) - 1))] = (uint32_t)k;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Similarity of undecided edges (GPUSCAN++/GPUScan.cu:174-222)
+// ---------------------------------------------------------------------------
+
+// Smallest c in [0, s] with (double)((c+2)^2) >= rhs, or s + 1 if none; the
+// reference's predicate `(cn+2)^2 < rhs` (int product, double compare) is then
+// `cn < c`, valid while (s+2)^2 < 2^31.
+__device__ __forceinline__ uint32_t min_common_threshold(double rhs, uint32_t s) {
+  if (!(rhs > 4.0)) return 0;  // also NaN: (double)lhs < NaN is false -> similar
+  double r = sqrt(rhs);
+  double c0 = ceil(r) - 2.0;
+  uint32_t c = c0 <= 0.0 ? 0u : (c0 >= (double)(s + 1) ? s + 1 : (uint32_t)c0);
+  while (c > 0 && (double)((uint64_t)(c + 1) * (uint64_t)(c + 1)) >= rhs) --c;
+  while (c <= s && (double)((uint64_t)(c + 2) * (uint64_t)(c + 2)) < rhs) ++c;
+  return c;
+}
+
+// The reference's predicate with its int32 wrap, for rows too long for the
+// exact threshold.
+__device__ __forceinline__ bool predicate_exact(uint32_t cn, double rhs) {
+  const uint32_t t = cn + 2u;
+  const int32_t lhs = (int32_t)(t * t);
+  return !((double)lhs < rhs);
+}
+
+constexpr uint32_t kMaxThresholdRow = 46338;  // (s + 2)^2 < 2^31
+
+// One wave per undecided edge (a, b), s = d(a) <= l = d(b): lanes take the
+// short row 64 elements at a time and binary-search each in the long row,
+// keeping a per-lane lower bound (each lane's keys increase).  The count is
+// checked against the exact threshold after every 64 elements: reached ->
+// similar, unreachable -> not similar.  Writes the flag of both directed
+// entries.
+__global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ adj,
+                                                            const uint32_t* __restrict__ src, double eps,
+                                                            uint8_t* __restrict__ flag,
+                                                            const uint32_t* __restrict__ ulist,
+                                                            uint64_t* __restrict__ cnt) {
+  const uint32_t lane = lane_id();
+  const uint64_t nU = cnt[kCntUEdges];
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  uint64_t probes = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < nU; i += waves) {
+    const uint32_t k = ulist[i];
+    const uint32_t b = src[k];
+    const uint32_t a = (uint32_t)adj[k];
+    const uint32_t ab = rp[a], ae = rp[a + 1], bb = rp[b], be = rp[b + 1];
+    const uint32_t s = ae - ab, l = be - bb;
+    const double rhs = eps * eps * (double)(int32_t)(s + 1u) * (double)(int32_t)(l + 1u);
+    const bool thresholded = s <= kMaxThresholdRow;
+    const uint32_t cstar = thresholded ? min_common_threshold(rhs, s) : 0u;
+    uint32_t cn = 0;
+    uint32_t lo = bb;
+    uint32_t off = 0;
+    bool decided = false, sim = false;
+    if (thresholded && cstar == 0) {
+      decided = true;
+      sim = true;
+    } else if (thresholded && cstar > s) {
+      decided = true;
+      sim = false;
+    }
+    while (!decided && off < s) {
+      bool hit = false;
+      if (off + lane < s) {
+        const int32_t x = adj[ab + off + lane];
+        uint32_t h = be;
+        while (lo < h) {
+          const uint32_t mid = (lo + h) >> 1;
+          if (adj[mid] < x)
+            lo = mid + 1;
+          else
+            h = mid;
+        }
+        hit = lo < be && adj[lo] == x;
+      }
+      cn += (uint32_t)__popcll(__ballot(hit));
+      off += kWave;
+      if (thresholded) {
+        const uint32_t rest = off < s ? s - off : 0u;
+        if (cn >= cstar) {
+          decided = true;
+          sim = true;
+        } else if (cn + rest < cstar) {
+          decided = true;
+          sim = false;
+        }
+      }
+    }
+    probes += off < s ? off : s;
+    if (!decided) sim = thresholded ? cn >= cstar : predicate_exact(cn, rhs);
+    if (lane == 0) {
+      const uint8_t f = sim ? kFlagY : kFlagN;
+      flag[k] = f;
+      // mirror entry: position of b in a's row
+      uint32_t x0 = ab, x1 = ae;
+      while (x0 < x1) {
+        const uint32_t mid = (x0 + x1) >> 1;
+        if (adj[mid] < (int32_t)b)
+          x0 = mid + 1;
+        else
+          x1 = mid;
+      }
+      if (x0 < ae && adj[x0] == (int32_t)b) flag[x0] = f;
+    }
+  }
+  if (lane == 0 && probes) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)probes);
+}
+
+// ---------------------------------------------------------------------------
+// Core check (GPUSCAN++/GPUScan.cu:115-131, :276-367)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_t* __restrict__ src,
+                                                       const uint8_t* __restrict__ flag, uint32_t* __restrict__ sd) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
+    const uint64_t k = base + lane;
+    const bool act = k < m;
+    const uint32_t row = act ? src[k] : 0xFFFFFFFFu;
+    const uint32_t y = act && flag[k] == kFlagY ? 1u : 0u;
+    const uint32_t sum = wave_seg_reduce(row, y, OpAdd());
+    if (wave_seg_head(row) && act && sum) atomicAdd(&sd[row], sum);
+  }
+}
+
+// core(x) <=> #similar neighbours >= mu (mu <= 0: >= 1).  The reference's
+// sd/ed state machine reaches exactly this fixed point: sd only counts
+// similar edges, ed only discounts dissimilar ones, and every edge of a
+// still-undecided vertex is evaluated.
+__global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t* __restrict__ sd, uint32_t thr,
+                                                      uint8_t* __restrict__ core, int32_t* __restrict__ label) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    const bool c = sd[x] >= thr;
+    core[x] = c;
+    label[x] = c ? (int32_t)x : kNoLabel;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Clusters: lock-free union-find, parents always smaller than children
+// (GPUSCAN++/GPUScan.cu:465-528, :632-652)
+// ---------------------------------------------------------------------------
+
+__device__ int32_t uf_find(int32_t* parent, int32_t x) {
+  int32_t p = ld_coherent(parent + x);
+  while (p != x) {
+    const int32_t gp = ld_coherent(parent + p);
+    if (gp != p) st_coherent(parent + x, gp);  // path splitting
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
+  for (;;) {
+    const int32_t ra = uf_find(parent, a), rb = uf_find(parent, b);
+    if (ra == rb) return;
+    const int32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+    if (atomicCAS(parent + hi, hi, lo) == hi) return;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __restrict__ src,
+                                                  const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
+                                                  const uint8_t* __restrict__ core, int32_t* parent) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
+    const int32_t v = adj[k];
+    const uint32_t u = src[k];
+    if ((uint32_t)v < u && flag[k] == kFlagY && core[u] && core[v]) uf_union(parent, (int32_t)u, v);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* __restrict__ core, int32_t* label) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride)
+    if (core[x]) label[x] = uf_find(label, (int32_t)x);
+}
+
+// ---------------------------------------------------------------------------
+// Borders (GPUSCAN++/GPUScan.cu:654-730): minimum candidate cluster id
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* __restrict__ src,
+                                                   const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
+                                                   const uint8_t* __restrict__ core, int32_t* label) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
+    const uint64_t k = base + lane;
+    const bool act = k < m;
+    uint32_t row = 0xFFFFFFFFu;
+    int32_t cand = kNoLabel;
+    if (act) {
+      row = src[k];
+      const int32_t c = adj[k];
+      if (!core[row] && core[c] && flag[k] == kFlagY) cand = label[c];
+    }
+    const int32_t best = wave_seg_reduce(row, cand, OpMin());
+    if (wave_seg_head(row) && act && best != kNoLabel) atomicMin(&label[row], best);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hubs and outliers (GPUSCAN++/GPUScan.cu:732-798)
+// ---------------------------------------------------------------------------
+
+// For every vertex still unlabelled: min and max of its neighbours' labels.
+// Two different labels <=> the reference finds a first labelled neighbour and
+// a later one with another label.
+__global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, const uint32_t* __restrict__ src,
+                                                     const int32_t* __restrict__ adj,
+                                                     const int32_t* __restrict__ label, int32_t* __restrict__ lo,
+                                                     int32_t* __restrict__ hi) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
+    const uint64_t k = base + lane;
+    const bool act = k < m;
+    uint32_t row = 0xFFFFFFFFu;
+    int32_t vmin = kNoLabel, vmax = -1;
+    if (act) {
+      row = src[k];
+      if (label[row] == kNoLabel) {
+        const int32_t ly = label[adj[k]];
+        if (ly != kNoLabel) {
+          vmin = ly;
+          vmax = ly;
+        }
+      }
+    }
+    vmin = wave_seg_reduce(row, vmin, OpMin());
+    vmax = wave_seg_reduce(row, vmax, OpMax());
+    if (wave_seg_head(row) && act && vmax >= 0) {
+      atomicMin(&lo[row], vmin);
+      atomicMax(&hi[row], vmax);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, const int32_t* __restrict__ lo,
+                                                     const int32_t* __restrict__ hi, const uint8_t* __restrict__ core,
+                                                     int32_t* __restrict__ label, uint64_t* __restrict__ cnt) {
+  uint32_t cores = 0, clusters = 0, borders = 0, hubs = 0, outliers = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    int32_t l = label[x];
+    if (core[x]) {
+      ++cores;
+      clusters += l == (int32_t)x;
+    } else if (l != kNoLabel) {
+      ++borders;
+    } else {
+      // mu <= 0: the reference classifies only role -2 vertices and there
+      // are none, so every non-core keeps role -1 (GPUScan.cu:978-984).
+      const bool hub = mu <= 0 || (lo[x] != kNoLabel && lo[x] != hi[x]);
+      l = hub ? -1 : -2;
+      label[x] = l;
+      hubs += hub;
+      outliers += !hub;
+    }
+  }
+  // wave reduction, then one atomic per wave
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    cores += __shfl_down(cores, off);
+    clusters += __shfl_down(clusters, off);
+    borders += __shfl_down(borders, off);
+    hubs += __shfl_down(hubs, off);
+    outliers += __shfl_down(outliers, off);
+  }
+  if (lane_id() == 0) {
+    if (cores) atomicAdd((unsigned long long*)&cnt[kCntCores], (unsigned long long)cores);
+    if (clusters) atomicAdd((unsigned long long*)&cnt[kCntClusters], (unsigned long long)clusters);
+    if (borders) atomicAdd((unsigned long long*)&cnt[kCntBorders], (unsigned long long)borders);
+    if (hubs) atomicAdd((unsigned long long*)&cnt[kCntHubs], (unsigned long long)hubs);
+    if (outliers) atomicAdd((unsigned long long*)&cnt[kCntOutliers], (unsigned long long)outliers);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint8_t* __restrict__ flag,
+                                                          uint64_t* __restrict__ cnt) {
+  uint32_t y = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) y += flag[k] == kFlagY;
+  for (int off = kWave / 2; off > 0; off >>= 1) y += __shfl_down(y, off);
+  if (lane_id() == 0 && y) atomicAdd((unsigned long long*)&cnt[kCntSimilar], (unsigned long long)y);
+}
+
+uint32_t grid_for(uint64_t items, uint32_t per_block) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > (1u << 20)) g = 1u << 20;
+  return (uint32_t)g;
+}
+
+}  // namespace
+
+void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
+  if (g->n == 0 || g->m == 0) return;
+  k_fill_src<<<grid_for(g->n, kBlock / kWave), kBlock, 0, s>>>(g->n, g->row_ptr, g->src);
+}
+
+void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt);
+}
+
+void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  if (g->m == 0) return;
+  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.cnt);
+}
+
+void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
+  if (g->m == 0) return;
+  const uint32_t blocks = (uint32_t)(cu_count > 0 ? cu_count : 256) * 8;
+  k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.cnt);
+}
+
+void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  if (g->m) k_core_count<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.sd);
+  const uint32_t thr = p.mu >= 1 ? (uint32_t)p.mu : 1u;
+  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, thr, b.core, b.label);
+}
+
+void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+  if (g->m) k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.label);
+  k_flatten<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.core, b.label);
+}
+
+void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+  if (g->m) k_border<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.label);
+}
+
+void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  if (g->m) k_classify<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.label, b.lo, b.hi);
+  k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, b.lo, b.hi, b.core, b.label, b.cnt);
+  if (g->m) k_count_similar<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, b.cnt);
+}
+
+}  // namespace sgc

@@ -1,0 +1,206 @@
+/*
+ * sgc.h - C ABI of the MI355X SCAN structural graph clusterer (libsgc.so).
+ *
+ * Every entry point returns an int status (SGC_OK on success) and never calls
+ * exit(); the message for the last failure on the calling thread is returned
+ * by sgc_last_error().  Pointers are plain host or device pointers; nothing in
+ * this header depends on C++ or on any framework type.
+ *
+ * Label encoding is the reference's (GPUSCAN++/GPUScan.cu:124-127, :649,
+ * :780): label >= 0 is the cluster id (the minimum core id of the cluster),
+ * -1 a hub, -2 an outlier.
+ */
+#ifndef SGC_H
+#define SGC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGC_OK 0
+#define SGC_EINVAL 1  /* bad argument */
+#define SGC_EIO 2     /* file missing, unreadable or truncated */
+#define SGC_EFORMAT 3 /* input violates the CSR contract */
+#define SGC_ENOMEM 4  /* host/device allocation failed or budget too small */
+#define SGC_EDEVICE 5 /* HIP runtime error */
+#define SGC_ERANGE 6  /* size beyond what this entry point supports */
+
+#define SGC_HUB (-1)
+#define SGC_OUTLIER (-2)
+
+/* Similarity flag values written per directed CSR entry. */
+#define SGC_SIM_NO 0
+#define SGC_SIM_YES 1
+
+const char* sgc_last_error(void);
+const char* sgc_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host CSR graphs                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sgc_host_graph sgc_host_graph;
+
+/* Replaces Graph::Graph(char* dir) (GPUSCAN++/Graph.cpp:147-153):
+ * ReadDegree (:157-174), ReadAdjacencyList (:176-223), CheckInputGraph
+ * (:270-288).  Reads <dir>/b_degree.bin = {int32 int_size, uint32 n,
+ * uint32 m, int32 deg[n]} and <dir>/b_adj.bin = int32 adj[sum(deg)].
+ * 64-bit extension: when sum(deg) >= 2^32 the header field holds
+ * sum(deg) mod 2^32.  Self loops and rows that are not strictly increasing
+ * fail with SGC_EFORMAT and the reference's message; ids >= n, header/size
+ * mismatches and (check_symmetry != 0) missing reverse entries also fail. */
+int sgc_graph_load(const char* dir, int check_symmetry, sgc_host_graph** out);
+
+/* Writes the two files read by sgc_graph_load (the format written by
+ * csr_convert/EdgeListToBinsConverter.cpp:85-121). */
+int sgc_graph_save(const sgc_host_graph* g, const char* dir);
+
+/* Copies a caller CSR (row_ptr[n+1], adj[m]) and validates it as
+ * sgc_graph_load does. */
+int sgc_graph_from_csr(uint64_t n, uint64_t m, const uint64_t* row_ptr, const int32_t* adj,
+                       int check_symmetry, sgc_host_graph** out);
+
+/* Builds a simple undirected graph from an edge list: drops self loops,
+ * symmetrises, removes duplicates, sorts rows. */
+int sgc_graph_from_edges(uint64_t n, uint64_t num_edges, const int32_t* src, const int32_t* dst,
+                         sgc_host_graph** out);
+
+/* Replaces csr_convert (EdgeListToBinsConverter.cpp:123-165): text edge
+ * list ("u v" per line, '#'/'%' comments) -> <out_dir>/b_degree.bin and
+ * b_adj.bin.  Unlike the reference converter it symmetrises and
+ * de-duplicates.  n = 1 + max id. */
+int sgc_graph_convert_edge_list(const char* txt_path, const char* out_dir);
+
+/* Seeded R-MAT: round(edge_factor * 2^scale) draws with quadrant
+ * probabilities a, b, c (d = 1-a-b-c) from the Philox4x32-10 stream, then
+ * self loops dropped, symmetrised, de-duplicated, rows sorted.  n = 2^scale. */
+int sgc_graph_rmat(int scale, double edge_factor, double a, double b, double c, uint64_t seed,
+                   sgc_host_graph** out);
+
+/* Seeded planted-partition graph with hubs.  Community sizes: i.i.d. Zipf
+ * draws floor(U^(-1/(zipf_exponent-1))) capped at n/20, scaled to sum to n
+ * (largest remainder, at least one vertex each); communities occupy
+ * consecutive id ranges.  Intra-community edge probability
+ * min(p_in, max_intra_degree/(size-1)); `inter_per_vertex` uniform draws per
+ * vertex into other communities; `hubs` distinct hub vertices each joined to
+ * every vertex with probability hub_fraction. */
+int sgc_graph_planted(uint64_t n, uint32_t communities, double zipf_exponent, double p_in,
+                      double max_intra_degree, uint32_t inter_per_vertex, uint32_t hubs,
+                      double hub_fraction, uint64_t seed, sgc_host_graph** out);
+
+/* The counter-based stream behind both generators: Philox4x32-10 of
+ * counter ctr[4] under key[2] (Salmon et al., SC'11), written to out[4]. */
+void sgc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+uint64_t sgc_graph_n(const sgc_host_graph* g);
+uint64_t sgc_graph_m(const sgc_host_graph* g);
+const uint64_t* sgc_graph_row_ptr(const sgc_host_graph* g);
+const int32_t* sgc_graph_adj(const sgc_host_graph* g);
+uint32_t sgc_graph_max_degree(const sgc_host_graph* g);
+/* FNV-1a 64 over n, m, row_ptr and adj (little-endian words). */
+uint64_t sgc_graph_digest(const sgc_host_graph* g);
+void sgc_graph_free(sgc_host_graph* g);
+
+/* Replaces getEidAndEdgeList (GPUSCAN++/Graph.cpp:52-141): orients every
+ * undirected edge from its lower (degree, id) endpoint to the other one.
+ * edge_u/edge_v[m/2] list the edges ordered by edge_u, then by edge_v;
+ * eid[m] maps each directed CSR entry to its undirected edge id.  Any output
+ * pointer may be NULL.  Requires m < 2^32. */
+int sgc_orient(const sgc_host_graph* g, int32_t* edge_u, int32_t* edge_v, uint32_t* eid);
+
+/* Work model of the similarity phase at `eps` (host, OpenMP): the number of
+ * edges the degree-only test leaves undecided and their algorithmic bytes
+ * sum 4 * (s + min(l, s * ceil(log2(l + 1)))) over those edges (s <= l the
+ * endpoint degrees): the short row streamed once, the long row streamed
+ * once or probed once per binary-search level, whichever is smaller. */
+int sgc_graph_similarity_work(const sgc_host_graph* g, double eps, uint64_t* u_edges, double* alg_bytes,
+                              double* short_elements);
+
+/* ------------------------------------------------------------------------ */
+/* Device context and device-resident graphs                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sgc_ctx sgc_ctx;
+
+/* One context per (device, stream).  The context owns a HIP stream created
+ * at the default priority and a grow-only workspace. */
+int sgc_ctx_create(int device, sgc_ctx** out);
+void sgc_ctx_destroy(sgc_ctx* ctx);
+int sgc_ctx_sync(sgc_ctx* ctx);
+/* Name of the device the context runs on (e.g. "AMD Instinct MI355X"). */
+const char* sgc_ctx_device_name(const sgc_ctx* ctx);
+
+typedef struct sgc_dgraph sgc_dgraph;
+
+/* Copies a host CSR into HBM (uint32 offsets; requires m < 2^32, else
+ * SGC_ERANGE - use sgc_scan_ooc for larger graphs). */
+int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out);
+void sgc_dgraph_free(sgc_dgraph* dg);
+uint64_t sgc_dgraph_n(const sgc_dgraph* dg);
+uint64_t sgc_dgraph_m(const sgc_dgraph* dg);
+
+/* ------------------------------------------------------------------------ */
+/* SCAN                                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  /* Device time from resident CSR to resident labels, hipEvent-timed on the
+   * context stream, and its phases. */
+  double ms_total;
+  double ms_prep;       /* orientation + degree-only pre-decision */
+  double ms_similarity; /* common-neighbour kernels */
+  double ms_core;       /* core check */
+  double ms_cluster;    /* union-find + flatten */
+  double ms_border;     /* border assignment */
+  double ms_classify;   /* hub / outlier */
+  uint64_t edges;       /* E = m/2 */
+  uint64_t u_edges;     /* edges the degree-only test could not decide */
+  uint64_t similar_edges;
+  uint64_t cores;
+  uint64_t clusters;
+  uint64_t borders;
+  uint64_t hubs;
+  uint64_t outliers;
+  double alg_bytes;      /* algorithmic bytes of the similarity kernels */
+  double probe_elements; /* list elements actually streamed by them */
+} sgc_stats;
+
+/* Replaces GPUScan(graph_t*, int*, int*, double, int) (GPUSCAN++/GPUScan.h:7,
+ * GPUScan.cu:800-1009) for a device-resident graph.  Outputs are host
+ * arrays, each optional (NULL = not copied back): labels[n] (int32),
+ * core[n] (1 = core), sim[m] (SGC_SIM_YES/NO per directed CSR entry).
+ * Threshold arithmetic is the reference's own FP64 expression
+ * (GPUScan.cu:143-156, :220-221).  Borders take the minimum candidate
+ * cluster id. */
+int sgc_scan(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* labels, uint8_t* core,
+             uint8_t* sim, sgc_stats* stats);
+
+/* Same pass with device output pointers (each optional); nothing is copied
+ * to the host and the call returns once the pass completed on the stream. */
+int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* d_labels,
+                    uint8_t* d_core, uint8_t* d_sim, sgc_stats* stats);
+
+/* Reference-layout graph (GPUSCAN++/Graph.h:13-20). */
+typedef struct {
+  long n;
+  long m;
+  int32_t* adj;
+  uint32_t* num_edges;
+  uint32_t* eid;
+} sgc_graph_t;
+
+/* Drop-in for int* GPUScan(graph_t* g, int* Edge_u, int* Edge_v, double eps,
+ * int miu) (GPUSCAN++/GPUScan.h:7).  Returns a malloc'd int32[n] the caller
+ * frees, or NULL on error (message in sgc_last_error()).  Edge_u/Edge_v/eid
+ * are accepted for signature compatibility and are not read: the device
+ * pass orients edges itself. */
+int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v, double eps, int32_t miu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGC_H */

@@ -1,0 +1,179 @@
+"""HIP path vs the oracle, through the C ABI (needs an MI355X).
+
+Bit-exact: per-vertex labels (cluster id = minimum core id, canonical
+minimum-candidate borders, -1 hub, -2 outlier), core flags and the
+similarity flag of every directed entry.
+"""
+from __future__ import annotations
+
+import json
+import os
+import random
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle_bind as ob
+from label_check import labels_from_flags
+
+import efficientstructuralgraphclusteringongpus_amd as sgc
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def run_gpu(ctx, g, eps, mu):
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        return sgc.scan(ctx, dg, eps, mu, want_sim=True)
+    finally:
+        dg.close()
+
+
+def assert_matches_oracle(ctx, g, eps, mu):
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    labels, core, sim, st = ob.orc_scan(rp, adj, eps, mu)
+    r = run_gpu(ctx, g, eps, mu)
+    assert np.array_equal(r.sim, sim), f"similarity differs at {np.nonzero(r.sim != sim)[0][:10]}"
+    assert np.array_equal(r.core, core)
+    assert np.array_equal(r.labels, labels), f"labels differ at {np.nonzero(r.labels != labels)[0][:10]}"
+    assert r.stats.cores == st["cores"]
+    assert r.stats.clusters == st["clusters"]
+    assert r.stats.similar_edges == st["similar_edges"]
+    assert r.stats.hubs == st["hubs"] and r.stats.outliers == st["outliers"] and r.stats.borders == st["borders"]
+    assert r.stats.u_edges == st["prep_u"]
+    return r
+
+
+def test_known_answer_graphs(ctx):
+    data = json.load(open(os.path.join(GOLDEN, "kat_graphs.json")))
+    for case in data["cases"]:
+        say(case["name"])
+        rp, adj = bf.csr(case["n"], [tuple(e) for e in case["edges"]])
+        g = sgc.HostGraph.from_csr(rp, adj)
+        r = run_gpu(ctx, g, case["eps"], case["mu"])
+        assert list(r.labels) == case["labels"], case["name"]
+        assert list(r.core) == case["core"], case["name"]
+        assert r.stats.cores == case["core_number"] and r.stats.clusters == case["cluster_number"]
+
+
+def test_random_small_graphs(ctx):
+    rng = random.Random(21)
+    for trial in range(60):
+        n = rng.randint(1, 300)
+        p = rng.choice([0.01, 0.05, 0.2, 0.5])
+        edges = [(u, v) for u in range(n) for v in range(u + 1, n) if rng.random() < p]
+        eps = rng.choice([0.0, 0.1, 0.25, 0.5, 0.65, 0.9, 1.0, 1.5])
+        mu = rng.choice([-2, 0, 1, 2, 3, 6])
+        say("random", trial, n, len(edges), eps, mu)
+        rp, adj = bf.csr(n, edges)
+        assert_matches_oracle(ctx, sgc.HostGraph.from_csr(rp, adj), eps, mu)
+
+
+def test_rmat16_default_config(ctx):
+    say("rmat16 eps=0.5 mu=5")
+    g = sgc.HostGraph.rmat(16, 16, seed=1)
+    r = assert_matches_oracle(ctx, g, 0.5, 5)
+    assert r.stats.cores > 0 and r.stats.clusters > 0
+
+
+@pytest.mark.parametrize("eps", [0.2, 0.4, 0.6, 0.8])
+@pytest.mark.parametrize("mu", [2, 5, 10])
+def test_rmat14_grid(ctx, eps, mu):
+    say("rmat14", eps, mu)
+    g = sgc.HostGraph.rmat(14, 16, seed=2)
+    assert_matches_oracle(ctx, g, eps, mu)
+
+
+def test_int32_wrap_hub_pair(ctx):
+    """Two hubs with 50000 common neighbours: (cn+2)^2 wraps in the reference's
+    int product and the hub-hub edge comes out dissimilar."""
+    say("wrap hub pair")
+    k = 50000
+    src = [0] + [0] * k + [1] * k + list(range(2, 2 + k - 1))
+    dst = [1] + list(range(2, 2 + k)) + list(range(2, 2 + k)) + list(range(3, 3 + k - 1))
+    g = sgc.HostGraph.from_edges(k + 2, src, dst)
+    for eps in (0.2, 0.5, 0.9):
+        r = assert_matches_oracle(ctx, g, eps, 2)
+        assert r.sim[0] == 0  # entry (0 -> 1)
+
+
+def test_empty_and_isolated(ctx):
+    say("edgeless")
+    g = sgc.HostGraph.from_csr([0, 0, 0, 0], [])
+    for mu in (0, 1, 3):
+        r = run_gpu(ctx, g, 0.5, mu)
+        assert list(r.labels) == ([-1] * 3 if mu <= 0 else [-2] * 3)
+
+
+def test_repeat_runs_identical(ctx):
+    say("determinism")
+    g = sgc.HostGraph.rmat(15, 16, seed=5)
+    dg = sgc.DeviceGraph(ctx, g)
+    a = sgc.scan(ctx, dg, 0.4, 3, want_sim=True)
+    b = sgc.scan(ctx, dg, 0.4, 3, want_sim=True)
+    dg.close()
+    assert np.array_equal(a.labels, b.labels) and np.array_equal(a.sim, b.sim)
+
+
+def test_rmat20_sampled_oracle_and_label_rules(ctx):
+    """Full-size consistency: similarity of a random 2% of entries against the
+    oracle, and labels re-derived from the GPU's own flags by the vectorised
+    label rules."""
+    say("rmat20")
+    g = sgc.HostGraph.rmat(20, 16, seed=1)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    r = run_gpu(ctx, g, 0.5, 5)
+    rng = np.random.default_rng(0)
+    sample = rng.choice(len(adj), size=len(adj) // 50, replace=False).astype(np.uint64)
+    _, ref = ob.orc_similarity_entries(rp, adj, 0.5, sample)
+    assert np.array_equal(r.sim[sample.astype(np.int64)], ref)
+    labels, core = labels_from_flags(rp, adj, r.sim, 5)
+    assert np.array_equal(core, r.core)
+    assert np.array_equal(labels, r.labels)
+
+
+def test_reference_entry_point(ctx):
+    """sgc_GPUScan: the reference's int* GPUScan(graph_t*, ...) contract."""
+    import ctypes as C
+
+    say("sgc_GPUScan")
+    g = sgc.HostGraph.rmat(12, 16, seed=8)
+    lib = sgc.load_library()
+    rp32 = np.array(g.row_ptr, dtype=np.uint32)
+    adj = np.array(g.adj, dtype=np.int32)
+    gt = sgc._GraphT(g.n, g.m, adj.ctypes.data_as(C.POINTER(C.c_int32)), rp32.ctypes.data_as(C.POINTER(C.c_uint32)),
+                     None)
+    out = lib.sgc_GPUScan(C.byref(gt), None, None, 0.5, 4)
+    assert out
+    got = np.ctypeslib.as_array(out, shape=(g.n,)).copy()
+    C.CDLL(None).free(C.cast(out, C.c_void_p))
+    labels, _, _, _ = ob.orc_scan(g.row_ptr, adj, 0.5, 4)
+    assert np.array_equal(got, labels)
+
+
+def test_cli_matches_reference_lines(tmp_path):
+    say("GPUScan CLI")
+    g = sgc.HostGraph.rmat(13, 16, seed=6)
+    folder = str(tmp_path / "g")
+    g.save(folder)
+    out = str(tmp_path / "labels.bin")
+    cli = os.path.join(os.path.dirname(sgc.library_path()), "GPUScan")
+    proc = subprocess.run([cli, folder, "0.5", "3", "--out", out], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stderr
+    labels, core, _, st = ob.orc_scan(g.row_ptr, g.adj, 0.5, 3)
+    assert f"core number: {st['cores']}" in proc.stderr
+    assert f"Cluster Number: {st['clusters']}" in proc.stderr
+    assert f"max degree: {g.max_degree}" in proc.stdout
+    raw = open(out, "rb").read()
+    assert np.array_equal(np.frombuffer(raw[: 4 * g.n], dtype=np.int32), labels)
+    assert np.array_equal(np.frombuffer(raw[4 * g.n:], dtype=np.uint8), core)
+    bad = subprocess.run([cli, folder], capture_output=True, text=True)
+    assert bad.returncode == 1 and "parameter eps and miu" in bad.stderr
