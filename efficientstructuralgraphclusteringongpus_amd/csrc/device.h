@@ -70,7 +70,8 @@ namespace sgc {
 struct PassBuffers {
   uint8_t* flag;     // m
   uint32_t* sd;      // n: similar-degree counts
-  int32_t* label;    // n: union-find parent, then cluster id
+  int32_t* parent;   // n: union-find forest over cores (parent < child)
+  int32_t* label;    // n: cluster id (roots written by the flatten kernel)
   int32_t* lo;       // n: classify min neighbour label
   int32_t* hi;       // n: classify max neighbour label
   uint8_t* core;     // n

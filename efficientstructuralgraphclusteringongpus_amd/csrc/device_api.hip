@@ -40,6 +40,7 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   };
   const size_t o_flag = take(m);
   const size_t o_sd = take(n * 4);
+  const size_t o_parent = take(n * 4);
   const size_t o_label = take(n * 4);
   const size_t o_lo = take(n * 4);
   const size_t o_hi = take(n * 4);
@@ -50,6 +51,7 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   char* base = (char*)ctx->work.ptr;
   pb->flag = (uint8_t*)(base + o_flag);
   pb->sd = (uint32_t*)(base + o_sd);
+  pb->parent = (int32_t*)(base + o_parent);
   pb->label = (int32_t*)(base + o_label);
   pb->lo = (int32_t*)(base + o_lo);
   pb->hi = (int32_t*)(base + o_hi);

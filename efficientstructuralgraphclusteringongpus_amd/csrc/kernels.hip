@@ -264,12 +264,14 @@ __global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_
 // similar edges, ed only discounts dissimilar ones, and every edge of a
 // still-undecided vertex is evaluated.
 __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t* __restrict__ sd, uint32_t thr,
-                                                      uint8_t* __restrict__ core, int32_t* __restrict__ label) {
+                                                      uint8_t* __restrict__ core, int32_t* __restrict__ parent,
+                                                      int32_t* __restrict__ label) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     const bool c = sd[x] >= thr;
     core[x] = c;
-    label[x] = c ? (int32_t)x : kNoLabel;
+    parent[x] = (int32_t)x;
+    label[x] = kNoLabel;
   }
 }
 
@@ -309,10 +311,13 @@ __global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* __restrict__ core, int32_t* label) {
+// Roots go to a separate array: the finds keep splitting paths in `parent`
+// (stores of ancestors only), which must not land on a finished label.
+__global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* __restrict__ core, int32_t* parent,
+                                                    int32_t* __restrict__ label) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride)
-    if (core[x]) label[x] = uf_find(label, (int32_t)x);
+    if (core[x]) label[x] = uf_find(parent, (int32_t)x);
 }
 
 // ---------------------------------------------------------------------------
@@ -456,12 +461,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
 void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   if (g->m) k_core_count<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.sd);
   const uint32_t thr = p.mu >= 1 ? (uint32_t)p.mu : 1u;
-  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, thr, b.core, b.label);
+  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, thr, b.core, b.parent, b.label);
 }
 
 void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  if (g->m) k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.label);
-  k_flatten<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.core, b.label);
+  if (g->m) k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent);
+  k_flatten<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.core, b.parent, b.label);
 }
 
 void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {

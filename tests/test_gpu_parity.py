@@ -81,7 +81,11 @@ def test_rmat16_default_config(ctx):
     say("rmat16 eps=0.5 mu=5")
     g = sgc.HostGraph.rmat(16, 16, seed=1)
     r = assert_matches_oracle(ctx, g, 0.5, 5)
-    assert r.stats.cores > 0 and r.stats.clusters > 0
+    # R-MAT at eps = 0.5 is almost structureless: 350 similar edges, no core
+    assert r.stats.similar_edges == 350 and r.stats.cores == 0
+    say("rmat16 eps=0.2 mu=2")
+    r = assert_matches_oracle(ctx, g, 0.2, 2)
+    assert r.stats.cores > 0 and r.stats.clusters > 0 and r.stats.borders > 0 and r.stats.hubs > 0
 
 
 @pytest.mark.parametrize("eps", [0.2, 0.4, 0.6, 0.8])
@@ -103,6 +107,18 @@ def test_int32_wrap_hub_pair(ctx):
     for eps in (0.2, 0.5, 0.9):
         r = assert_matches_oracle(ctx, g, eps, 2)
         assert r.sim[0] == 0  # entry (0 -> 1)
+
+
+def test_long_paths_and_cycles_are_one_cluster(ctx):
+    """Regression: deep union-find trees while many threads compress paths;
+    every core must end with its component's minimum id."""
+    say("long path / cycle")
+    for n in (1000, 100_000):
+        src = list(range(n - 1)) + [n - 1]
+        dst = list(range(1, n)) + [0]
+        g = sgc.HostGraph.from_edges(n, src, dst)
+        r = assert_matches_oracle(ctx, g, 0.4, 2)
+        assert r.stats.clusters == 1 and set(r.labels.tolist()) == {0}
 
 
 def test_empty_and_isolated(ctx):
