@@ -144,6 +144,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_ctx_destroy", None, [_VP])
     _sig(lib, "sgc_ctx_sync", C.c_int, [_VP])
     _sig(lib, "sgc_ctx_device_name", C.c_char_p, [_VP])
+    _sig(lib, "sgc_ctx_set_similarity_kernel", C.c_int, [_VP, C.c_int])
     _sig(lib, "sgc_dgraph_upload", C.c_int, [_VP, _VP, _P(_VP)])
     _sig(lib, "sgc_dgraph_free", None, [_VP])
     _sig(lib, "sgc_dgraph_n", C.c_uint64, [_VP])
@@ -321,6 +322,12 @@ class Context:
 
     def sync(self) -> None:
         _check(self._lib.sgc_ctx_sync(self._h))
+
+    SIM_TIERED = 0
+    SIM_WAVE_PER_EDGE = 1
+
+    def set_similarity_kernel(self, kind: int) -> None:
+        _check(self._lib.sgc_ctx_set_similarity_kernel(self._h, int(kind)))
 
 
 class DeviceGraph:

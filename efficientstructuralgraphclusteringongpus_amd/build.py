@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["graph_host.cpp"]
-HIP_SOURCES = ["kernels.hip", "device_api.hip", "ooc.hip", "serve.hip", "batch.hip"]
+HIP_SOURCES = ["kernels.hip", "similarity.hip", "device_api.hip", "ooc.hip", "serve.hip", "batch.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, "-Wall", "-Wno-unused-function"]
 HOST_FLAGS = COMMON + ["-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),

@@ -19,8 +19,11 @@ namespace sgc {
 constexpr int kWave = 64;
 constexpr int32_t kNoLabel = 0x7FFFFFFF;
 
-// Per-edge flag byte stored for every directed CSR entry.
-enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagU = 2 };
+// Per-edge flag byte stored for every directed CSR entry.  Undecided edges are
+// marked in both rows: kFlagUDown in the higher (degree, id) endpoint's row,
+// which owns the computation, kFlagUUp in the other row, which receives the
+// result.  After the similarity phase every flag is kFlagN or kFlagY.
+enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagUUp = 2, kFlagUDown = 3 };
 
 // Counters written by the kernels (one 64-bit slot each).
 enum Counter {
@@ -32,8 +35,16 @@ enum Counter {
   kCntHubs,
   kCntOutliers,
   kCntProbes,
+  kCntUnits1,   // wave-tier work units
+  kCntUnits2,   // workgroup-tier work units
+  kCntQueue1,   // next wave-tier unit to claim
+  kCntQueue2,   // next workgroup-tier unit to claim
+  kCntQueueE,   // next edge for the one-wave-per-edge kernel
   kCntSlots = 16
 };
+
+// Similarity kernels selectable per context.
+enum SimImpl { kSimTiered = 0, kSimWavePerEdge = 1 };
 
 // Grow-only device allocation owned by a context.
 struct DevBuf {
@@ -52,6 +63,7 @@ struct sgc_ctx {
   sgc::DevBuf counters;  // kCntSlots x uint64
   uint64_t* host_counters = nullptr;  // pinned mirror
   hipEvent_t ev[8] = {};
+  int sim_impl = sgc::kSimTiered;
 };
 
 struct sgc_dgraph {
@@ -75,7 +87,14 @@ struct PassBuffers {
   int32_t* lo;       // n: classify min neighbour label
   int32_t* hi;       // n: classify max neighbour label
   uint8_t* core;     // n
-  uint32_t* ulist;   // m/2: undecided edges, by entry index in the higher row
+  uint32_t* cnt_u;   // n: undecided edges owned by each vertex (kFlagUDown entries)
+  uint64_t* work_u;  // n: sum of the short-row lengths of those edges
+  uint32_t* fill;    // n: scatter cursors
+  uint32_t* ustart;  // n + 1: exclusive scan of cnt_u
+  uint32_t* scan_tmp;  // block sums of the scan
+  uint32_t* ulist;   // m: owned undecided entries, grouped by owner
+  uint2* units1;     // wave-tier units: [begin, end) ranges of ulist
+  uint2* units2;     // workgroup-tier units
   uint64_t* cnt;     // kCntSlots
 };
 
@@ -84,11 +103,21 @@ struct ScanParams {
   int32_t mu;
 };
 
-// Kernel launchers (kernels.hip).  All run on `s` and never synchronise.
+// Units per tier are bounded by the number of owned undecided edges, which is
+// at most m; this is the capacity each unit array is allocated with.
+inline size_t unit_capacity(uint64_t m) { return (size_t)m + 1; }
+// Words of scratch the exclusive scan needs for `nitems` inputs.
+size_t scan_tmp_words(uint64_t nitems);
+
+// Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
+// synchronise.
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
-void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
+void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
+                       hipStream_t s);
+void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
 void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
 void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);

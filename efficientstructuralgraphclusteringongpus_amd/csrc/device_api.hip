@@ -45,7 +45,14 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   const size_t o_lo = take(n * 4);
   const size_t o_hi = take(n * 4);
   const size_t o_core = take(n);
+  const size_t o_cnt_u = take(n * 4);
+  const size_t o_work_u = take(n * 8);
+  const size_t o_fill = take(n * 4);
+  const size_t o_ustart = take((n + 1) * 4);
+  const size_t o_scan = take(scan_tmp_words(n) * 4);
   const size_t o_ulist = take(m * 4);
+  const size_t o_units1 = take(unit_capacity(m) * sizeof(uint2));
+  const size_t o_units2 = take(unit_capacity(m) * sizeof(uint2));
   int rc = ensure(ctx->work, off);
   if (rc) return rc;
   char* base = (char*)ctx->work.ptr;
@@ -56,7 +63,14 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   pb->lo = (int32_t*)(base + o_lo);
   pb->hi = (int32_t*)(base + o_hi);
   pb->core = (uint8_t*)(base + o_core);
+  pb->cnt_u = (uint32_t*)(base + o_cnt_u);
+  pb->work_u = (uint64_t*)(base + o_work_u);
+  pb->fill = (uint32_t*)(base + o_fill);
+  pb->ustart = (uint32_t*)(base + o_ustart);
+  pb->scan_tmp = (uint32_t*)(base + o_scan);
   pb->ulist = (uint32_t*)(base + o_ulist);
+  pb->units1 = (uint2*)(base + o_units1);
+  pb->units2 = (uint2*)(base + o_units2);
   pb->cnt = (uint64_t*)ctx->counters.ptr;
   return SGC_OK;
 }
@@ -81,7 +95,7 @@ extern "C" int sgc_ctx_create(int device, sgc_ctx** out) {
   ctx->device = device;
   hipDeviceProp_t prop;
   SGC_HIP(hipGetDeviceProperties(&prop, device));
-  strncpy(ctx->name, prop.name, sizeof(ctx->name) - 1);
+  strncpy(ctx->name, prop.name[0] ? prop.name : prop.gcnArchName, sizeof(ctx->name) - 1);
   ctx->cu_count = prop.multiProcessorCount;
   SGC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   int rc = ensure(ctx->counters, kCntSlots * sizeof(uint64_t));
@@ -117,6 +131,15 @@ extern "C" int sgc_ctx_sync(sgc_ctx* ctx) {
 }
 
 extern "C" const char* sgc_ctx_device_name(const sgc_ctx* ctx) { return ctx ? ctx->name : ""; }
+
+extern "C" int sgc_ctx_set_similarity_kernel(sgc_ctx* ctx, int kind) {
+  clear_error();
+  if (!ctx) return fail(SGC_EINVAL, "null context");
+  if (kind != SGC_SIM_KERNEL_TIERED && kind != SGC_SIM_KERNEL_WAVE_PER_EDGE)
+    return fail(SGC_EINVAL, "unknown similarity kernel %d", kind);
+  ctx->sim_impl = kind == SGC_SIM_KERNEL_TIERED ? kSimTiered : kSimWavePerEdge;
+  return SGC_OK;
+}
 
 extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out) {
   clear_error();
@@ -191,8 +214,9 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
   SGC_HIP(hipEventRecord(ctx->ev[0], s));
   launch_init(g, *pb, s);
   launch_prep(g, *pb, p, s);
+  launch_plan(g, *pb, s);
   SGC_HIP(hipEventRecord(ctx->ev[1], s));
-  launch_similarity(g, *pb, p, ctx->cu_count, s);
+  launch_similarity(g, *pb, p, ctx->cu_count, ctx->sim_impl, s);
   SGC_HIP(hipEventRecord(ctx->ev[2], s));
   launch_core(g, *pb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[3], s));

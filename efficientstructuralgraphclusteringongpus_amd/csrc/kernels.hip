@@ -1,59 +1,16 @@
-// SCAN kernels for gfx950 (wave64).  Every kernel is a grid-stride loop over
-// vertices or directed CSR entries; per-row aggregation inside a wave uses a
-// segmented shuffle reduction (entries of one row are consecutive lanes) so
-// each row issues at most one atomic per wave.
+// SCAN kernels for gfx950 (wave64) after the similarity phase, plus graph
+// preparation and the one-wave-per-edge similarity kernel.  Every kernel is a
+// grid-stride loop over vertices or directed CSR entries; per-row aggregation
+// inside a wave uses a segmented shuffle reduction (entries of one row are
+// consecutive lanes) so each row issues at most one atomic per wave.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "device_util.h"
 
 namespace sgc {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
-
-__device__ __forceinline__ uint32_t deg_of(const uint32_t* rp, uint32_t x) { return rp[x + 1] - rp[x]; }
-
-// Loads that bypass the non-coherent per-CU cache: union-find reads parents
-// that other CUs are changing.
-__device__ __forceinline__ int32_t ld_coherent(const int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_coherent(int32_t* p, int32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Segmented suffix reduction over a wave: lanes with equal `row` form
-// contiguous segments; on return the first lane of each segment holds the
-// reduction of its segment.  All 64 lanes must call it.
-template <typename T, typename Op>
-__device__ __forceinline__ T wave_seg_reduce(uint32_t row, T val, Op op) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    T ov = __shfl_down(val, off);
-    uint32_t orow = __shfl_down(row, off);
-    if (lane + off < kWave && orow == row) val = op(val, ov);
-  }
-  return val;
-}
-
-__device__ __forceinline__ bool wave_seg_head(uint32_t row) {
-  uint32_t prev = __shfl_up(row, 1);
-  return lane_id() == 0 || prev != row;
-}
-
-struct OpAdd {
-  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
-};
-struct OpMin {
-  __device__ int32_t operator()(int32_t a, int32_t b) const { return a < b ? a : b; }
-};
-struct OpMax {
-  __device__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
-};
 
 // ---------------------------------------------------------------------------
 // Graph preparation
@@ -75,95 +32,32 @@ __global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t*
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
-                                                 int32_t* __restrict__ hi, uint64_t* __restrict__ cnt) {
+                                                 int32_t* __restrict__ hi, uint32_t* __restrict__ cnt_u,
+                                                 uint64_t* __restrict__ work_u, uint32_t* __restrict__ fill,
+                                                 uint64_t* __restrict__ cnt) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     sd[x] = 0;
     lo[x] = kNoLabel;
     hi[x] = -1;
+    cnt_u[x] = 0;
+    work_u[x] = 0;
+    fill[x] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
-// Degree-only pre-decision (GPUSCAN++/GPUScan.cu:133-172)
+// Similarity, one wave per undecided edge (GPUSCAN++/GPUScan.cu:174-222)
 // ---------------------------------------------------------------------------
-
-// For every directed entry (u -> v): orient the edge (a = lower (degree, id)
-// endpoint, GPUSCAN++/Graph.cpp:75-83), apply the reference's two degree
-// tests in its own operand order, and store N / Y / U.  Undecided edges are
-// listed once, by their entry in the higher endpoint's row.
-__global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rp,
-                                                 const int32_t* __restrict__ adj, const uint32_t* __restrict__ src,
-                                                 double eps, uint8_t* __restrict__ flag,
-                                                 uint32_t* __restrict__ ulist, uint64_t* __restrict__ cnt) {
-  const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    bool listed = false;
-    if (k < m) {
-      const uint32_t u = src[k];
-      const uint32_t v = (uint32_t)adj[k];
-      const uint32_t du = deg_of(rp, u), dv = deg_of(rp, v);
-      const bool up = dv > du || (dv == du && v > u);
-      const uint32_t da = up ? du : dv, db = up ? dv : du;
-      const double A = (double)(da + 1u);
-      const double B = (double)(db + 1u);
-      uint8_t f;
-      if (A < B * eps * eps)
-        f = kFlagN;
-      else if (eps * eps * A * B <= 4.0)
-        f = kFlagY;
-      else
-        f = kFlagU;
-      flag[k] = f;
-      listed = f == kFlagU && !up;
-    }
-    const uint64_t mask = __ballot(listed);
-    if (mask) {
-      const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t pos = 0;
-      if (lane == leader) pos = (uint32_t)atomicAdd((unsigned long long*)&cnt[kCntUEdges], (unsigned long long)__popcll(mask));
-      pos = __shfl(pos, leader);
-      if (listed) ulist[pos + __popcll(mask & ((1ull << lane) - 1))] = (uint32_t)k;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Similarity of undecided edges (GPUSCAN++/GPUScan.cu:174-222)
-// ---------------------------------------------------------------------------
-
-// Smallest c in [0, s] with (double)((c+2)^2) >= rhs, or s + 1 if none; the
-// reference's predicate `(cn+2)^2 < rhs` (int product, double compare) is then
-// `cn < c`, valid while (s+2)^2 < 2^31.
-__device__ __forceinline__ uint32_t min_common_threshold(double rhs, uint32_t s) {
-  if (!(rhs > 4.0)) return 0;  // also NaN: (double)lhs < NaN is false -> similar
-  double r = sqrt(rhs);
-  double c0 = ceil(r) - 2.0;
-  uint32_t c = c0 <= 0.0 ? 0u : (c0 >= (double)(s + 1) ? s + 1 : (uint32_t)c0);
-  while (c > 0 && (double)((uint64_t)(c + 1) * (uint64_t)(c + 1)) >= rhs) --c;
-  while (c <= s && (double)((uint64_t)(c + 2) * (uint64_t)(c + 2)) < rhs) ++c;
-  return c;
-}
-
-// The reference's predicate with its int32 wrap, for rows too long for the
-// exact threshold.
-__device__ __forceinline__ bool predicate_exact(uint32_t cn, double rhs) {
-  const uint32_t t = cn + 2u;
-  const int32_t lhs = (int32_t)(t * t);
-  return !((double)lhs < rhs);
-}
-
-constexpr uint32_t kMaxThresholdRow = 46338;  // (s + 2)^2 < 2^31
 
 // One wave per undecided edge (a, b), s = d(a) <= l = d(b): lanes take the
 // short row 64 elements at a time and binary-search each in the long row,
 // keeping a per-lane lower bound (each lane's keys increase).  The count is
 // checked against the exact threshold after every 64 elements: reached ->
 // similar, unreachable -> not similar.  Writes the flag of both directed
-// entries.
+// entries.  Kept beside the tiered kernels as an independent second
+// implementation (selectable per context).
 __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __restrict__ rp,
                                                             const int32_t* __restrict__ adj,
                                                             const uint32_t* __restrict__ src, double eps,
@@ -180,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
     const uint32_t a = (uint32_t)adj[k];
     const uint32_t ab = rp[a], ae = rp[a + 1], bb = rp[b], be = rp[b + 1];
     const uint32_t s = ae - ab, l = be - bb;
-    const double rhs = eps * eps * (double)(int32_t)(s + 1u) * (double)(int32_t)(l + 1u);
+    const double rhs = predicate_rhs(eps, s, l);
     const bool thresholded = s <= kMaxThresholdRow;
     const uint32_t cstar = thresholded ? min_common_threshold(rhs, s) : 0u;
     uint32_t cn = 0;
@@ -223,19 +117,11 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
     }
     probes += off < s ? off : s;
     if (!decided) sim = thresholded ? cn >= cstar : predicate_exact(cn, rhs);
+    const uint32_t mpos = wave_lower_bound(adj, ab, ae, (int32_t)b);
     if (lane == 0) {
       const uint8_t f = sim ? kFlagY : kFlagN;
       flag[k] = f;
-      // mirror entry: position of b in a's row
-      uint32_t x0 = ab, x1 = ae;
-      while (x0 < x1) {
-        const uint32_t mid = (x0 + x1) >> 1;
-        if (adj[mid] < (int32_t)b)
-          x0 = mid + 1;
-        else
-          x1 = mid;
-      }
-      if (x0 < ae && adj[x0] == (int32_t)b) flag[x0] = f;
+      if (mpos < ae && adj[mpos] == (int32_t)b) flag[mpos] = f;
     }
   }
   if (lane == 0 && probes) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)probes);
@@ -429,12 +315,6 @@ __global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint
   if (lane_id() == 0 && y) atomicAdd((unsigned long long*)&cnt[kCntSimilar], (unsigned long long)y);
 }
 
-uint32_t grid_for(uint64_t items, uint32_t per_block) {
-  uint64_t g = (items + per_block - 1) / per_block;
-  if (g < 1) g = 1;
-  if (g > (1u << 20)) g = 1u << 20;
-  return (uint32_t)g;
-}
 
 }  // namespace
 
@@ -444,15 +324,10 @@ void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
 }
 
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt);
+  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt_u, b.work_u, b.fill, b.cnt);
 }
 
-void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
-  if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.cnt);
-}
-
-void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
+void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
   if (g->m == 0) return;
   const uint32_t blocks = (uint32_t)(cu_count > 0 ? cu_count : 256) * 8;
   k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.cnt);

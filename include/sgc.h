@@ -133,6 +133,16 @@ int sgc_ctx_sync(sgc_ctx* ctx);
 /* Name of the device the context runs on (e.g. "AMD Instinct MI355X"). */
 const char* sgc_ctx_device_name(const sgc_ctx* ctx);
 
+/* Common-neighbour kernels of the similarity phase.  TIERED (default): owner
+ * rows staged in LDS hash tables, short rows streamed against them.
+ * WAVE_PER_EDGE: one wavefront per undecided edge, binary search of the short
+ * row's elements in the long row (GPUSCAN++/GPUScan.cu:174-222 restated for
+ * wave64); kept as an independent second implementation.  Results are
+ * identical. */
+#define SGC_SIM_KERNEL_TIERED 0
+#define SGC_SIM_KERNEL_WAVE_PER_EDGE 1
+int sgc_ctx_set_similarity_kernel(sgc_ctx* ctx, int kind);
+
 typedef struct sgc_dgraph sgc_dgraph;
 
 /* Copies a host CSR into HBM (uint32 offsets; requires m < 2^32, else

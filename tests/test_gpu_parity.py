@@ -96,6 +96,44 @@ def test_rmat14_grid(ctx, eps, mu):
     assert_matches_oracle(ctx, g, eps, mu)
 
 
+@pytest.mark.parametrize("scale,ef,eps", [(16, 16, 0.2), (18, 16, 0.5), (17, 32, 0.3)])
+def test_tiered_and_wave_kernels_agree(ctx, scale, ef, eps):
+    """Two independent similarity kernels (LDS-hash tiers; one wave per edge
+    with binary search) give identical flags; a sample is checked against the
+    oracle too."""
+    say("kernels agree", scale, ef, eps)
+    g = sgc.HostGraph.rmat(scale, ef, seed=3)
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        ctx.set_similarity_kernel(sgc.Context.SIM_WAVE_PER_EDGE)
+        a = sgc.scan(ctx, dg, eps, 3, want_sim=True)
+        ctx.set_similarity_kernel(sgc.Context.SIM_TIERED)
+        b = sgc.scan(ctx, dg, eps, 3, want_sim=True)
+    finally:
+        ctx.set_similarity_kernel(sgc.Context.SIM_TIERED)
+        dg.close()
+    assert np.array_equal(a.sim, b.sim)
+    assert np.array_equal(a.labels, b.labels)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    sample = np.random.default_rng(1).choice(len(adj), size=min(len(adj), 200_000), replace=False)
+    _, ref = ob.orc_similarity_entries(rp, adj, eps, sample.astype(np.uint64))
+    assert np.array_equal(b.sim[sample], ref)
+
+
+def test_planted_partition_hubs(ctx):
+    """Hub-heavy graph: 64-vertex-scale version of the planted-partition
+    workload (hub rows take the segmented workgroup tier)."""
+    say("planted")
+    g = sgc.HostGraph.planted(n=200_000, communities=2_000, hubs=16, hub_fraction=0.1, seed=2)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    r = run_gpu(ctx, g, 0.5, 4)
+    sample = np.random.default_rng(2).choice(len(adj), size=300_000, replace=False).astype(np.uint64)
+    _, ref = ob.orc_similarity_entries(rp, adj, 0.5, sample)
+    assert np.array_equal(r.sim[sample.astype(np.int64)], ref)
+    labels, core = labels_from_flags(rp, adj, r.sim, 4)
+    assert np.array_equal(labels, r.labels) and np.array_equal(core, r.core)
+
+
 def test_int32_wrap_hub_pair(ctx):
     """Two hubs with 50000 common neighbours: (cn+2)^2 wraps in the reference's
     int product and the hub-hub edge comes out dissimilar."""
