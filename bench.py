@@ -130,7 +130,9 @@ def main() -> int:
     wall = time.perf_counter() - t0
     for eps, mu, st in stats:
         log(f"eps={eps} mu={mu}: pass {st.ms_total:.2f} ms, similarity {st.ms_similarity:.2f} ms, "
-            f"U={st.u_edges} cores={st.cores} clusters={st.clusters}")
+            f"U={st.u_edges} streamed={st.probe_elements:.3g} "
+            f"({st.probe_elements * 4 / (st.ms_similarity * 1e-3) / 1e9:.0f} GB/s) cores={st.cores} "
+            f"clusters={st.clusters}")
 
     ms_per_step = wall / args.steps * 1e3
     value = E * args.steps / wall / 1e9
@@ -175,6 +177,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "alg_bytes_per_launch": alg_bytes / len(stats),
+            "streamed_bytes_per_launch": sum(4 * st.probe_elements for _, _, st in stats) / len(stats),
             "avg_launch_ms": sim_ms / len(stats),
         },
         "cpu_baseline": cpu,

@@ -56,7 +56,7 @@ class _Stats(C.Structure):
         ("edges", C.c_uint64), ("u_edges", C.c_uint64), ("similar_edges", C.c_uint64),
         ("cores", C.c_uint64), ("clusters", C.c_uint64), ("borders", C.c_uint64),
         ("hubs", C.c_uint64), ("outliers", C.c_uint64),
-        ("alg_bytes", C.c_double), ("probe_elements", C.c_double),
+        ("alg_bytes", C.c_double), ("probe_elements", C.c_double), ("debug", C.c_double * 10),
     ]
 
 
@@ -84,10 +84,13 @@ class ScanStats:
     outliers: int
     alg_bytes: float
     probe_elements: float
+    debug: tuple
 
     @classmethod
     def _from(cls, s: _Stats) -> "ScanStats":
-        return cls(**{f.name: getattr(s, f.name) for f in fields(cls)})
+        vals = {f.name: getattr(s, f.name) for f in fields(cls)}
+        vals["debug"] = tuple(s.debug)
+        return cls(**vals)
 
 
 @dataclass

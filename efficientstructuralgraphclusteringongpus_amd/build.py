@@ -49,11 +49,16 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError("build step failed:\n  " + " ".join(cmd) + "\n" + out)
 
 
+def _extra_defines() -> list[str]:
+    """-D flags from SGC_BUILD_DEFINES (comma separated), for debug builds."""
+    return ["-D" + d for d in os.environ.get("SGC_BUILD_DEFINES", "").split(",") if d]
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
     if src.endswith(".hip"):
-        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+        _run([HIPCC, *HIP_FLAGS, *_extra_defines(), "-c", path, "-o", obj])
     else:
         _run([CXX, *HOST_FLAGS, "-c", path, "-o", obj])
     return obj

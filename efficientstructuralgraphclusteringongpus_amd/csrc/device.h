@@ -40,7 +40,8 @@ enum Counter {
   kCntQueue1,   // next wave-tier unit to claim
   kCntQueue2,   // next workgroup-tier unit to claim
   kCntQueueE,   // next edge for the one-wave-per-edge kernel
-  kCntSlots = 16
+  kCntTimer0 = 16,  // phase timers of debug builds (SGC_PHASE_TIMERS)
+  kCntSlots = 32
 };
 
 // Similarity kernels selectable per context.

@@ -248,6 +248,7 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
     st->hubs = c[kCntHubs];
     st->outliers = c[kCntOutliers];
     st->probe_elements = (double)c[kCntProbes];
+    for (int i = 0; i < 10; ++i) st->debug[i] = (double)c[kCntTimer0 + i];
   }
   return SGC_OK;
 }

@@ -33,9 +33,11 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Segmented suffix reduction over a wave: lanes with equal `row` form
-// contiguous segments; on return the first lane of each segment holds the
-// reduction of its segment.  All 64 lanes must call it.
+// Segmented suffix reduction over a wave: `row` must be non-decreasing across
+// lanes (equal rows then form contiguous segments; a lane compares only with
+// the lane `off` away, so an interleaved pattern u, v, u would merge); on
+// return the first lane of each segment holds the reduction of its segment.
+// All 64 lanes must call it.
 template <typename T, typename Op>
 __device__ __forceinline__ T wave_seg_reduce(uint32_t row, T val, Op op) {
   const uint32_t lane = lane_id();
@@ -98,11 +100,29 @@ __device__ __forceinline__ uint32_t wave_lower_bound(const int32_t* __restrict__
   return lo + (uint32_t)__popcll(__ballot(v < key));
 }
 
+// Grid for a grid-stride loop: enough blocks to fill every CU several times
+// over, few enough that per-block atomics on shared counters stay cheap.
 inline uint32_t grid_for(uint64_t items, uint32_t per_block) {
   uint64_t g = (items + per_block - 1) / per_block;
   if (g < 1) g = 1;
-  if (g > (1u << 20)) g = 1u << 20;
+  if (g > 4096) g = 4096;
   return (uint32_t)g;
+}
+
+// Adds v over the whole block to *dst with one atomic (all threads call it).
+__device__ __forceinline__ void block_atomic_add(unsigned long long* dst, unsigned long long v) {
+  __shared__ unsigned long long part[1024 / kWave];
+  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < (blockDim.x + kWave - 1) / kWave; ++i) t += part[i];
+    if (t) atomicAdd(dst, t);
+  }
+  __syncthreads();
 }
 
 // Smallest c in [0, s] with (double)((c+2)^2) >= rhs, or s + 1 if none; the

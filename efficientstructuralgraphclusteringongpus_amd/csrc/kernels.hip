@@ -289,21 +289,11 @@ __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, con
       outliers += !hub;
     }
   }
-  // wave reduction, then one atomic per wave
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    cores += __shfl_down(cores, off);
-    clusters += __shfl_down(clusters, off);
-    borders += __shfl_down(borders, off);
-    hubs += __shfl_down(hubs, off);
-    outliers += __shfl_down(outliers, off);
-  }
-  if (lane_id() == 0) {
-    if (cores) atomicAdd((unsigned long long*)&cnt[kCntCores], (unsigned long long)cores);
-    if (clusters) atomicAdd((unsigned long long*)&cnt[kCntClusters], (unsigned long long)clusters);
-    if (borders) atomicAdd((unsigned long long*)&cnt[kCntBorders], (unsigned long long)borders);
-    if (hubs) atomicAdd((unsigned long long*)&cnt[kCntHubs], (unsigned long long)hubs);
-    if (outliers) atomicAdd((unsigned long long*)&cnt[kCntOutliers], (unsigned long long)outliers);
-  }
+  block_atomic_add((unsigned long long*)&cnt[kCntCores], cores);
+  block_atomic_add((unsigned long long*)&cnt[kCntClusters], clusters);
+  block_atomic_add((unsigned long long*)&cnt[kCntBorders], borders);
+  block_atomic_add((unsigned long long*)&cnt[kCntHubs], hubs);
+  block_atomic_add((unsigned long long*)&cnt[kCntOutliers], outliers);
 }
 
 __global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint8_t* __restrict__ flag,
@@ -311,8 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint
   uint32_t y = 0;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) y += flag[k] == kFlagY;
-  for (int off = kWave / 2; off > 0; off >>= 1) y += __shfl_down(y, off);
-  if (lane_id() == 0 && y) atomicAdd((unsigned long long*)&cnt[kCntSimilar], (unsigned long long)y);
+  block_atomic_add((unsigned long long*)&cnt[kCntSimilar], y);
 }
 
 

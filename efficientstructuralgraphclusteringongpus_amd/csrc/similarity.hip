@@ -10,17 +10,16 @@
 // 3. k_plan: each owner's edges are cut into units of bounded work.
 //    Owners with d(b) <= kT1MaxDeg go to the wave tier, the rest to the
 //    workgroup tier.
-// 4. Counting: the owner's row N(b) is put into an LDS hash table and each
-//    owned edge streams the short row N(a) (4 coalesced loads per lane per
-//    step) probing the table.  Every 256 elements the running count is
-//    compared with the exact threshold of the reference's predicate:
-//    reached -> similar, unreachable -> not similar.  The result is written
-//    to both directed entries (the position of b in N(a) comes from the
-//    stream itself or from a 64-way wave search).
-//    Wave tier: each wave owns a private table (<= 2048 slots) and works one
-//    unit at a time.  Workgroup tier: 1024 threads share a 32768-slot table;
-//    rows longer than 16384 are split into value segments, each edge's short
-//    row is streamed segment by segment with its count kept in LDS.
+// 4. Counting: the owner's row N(b) (or one value segment of it) is put
+//    into an LDS hash table; the short rows N(a) of the unit's edges are cut
+//    into fixed-size items that every wave of the workgroup claims
+//    dynamically (16 coalesced loads per lane in flight for the large tier).
+//    Per edge the running (hits, consumed) pair decides early: reached ->
+//    similar, unreachable -> not similar (the reference's predicate in its
+//    exact threshold form).  The result is written to both directed entries.
+//    Small tier: owners of degree <= 1024, 256-thread workgroups.  Large
+//    tier: 1024-thread workgroups, 32768-slot table, rows longer than 16384
+//    split into value segments.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -30,17 +29,14 @@ namespace sgc {
 
 namespace {
 
+
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kT1MaxDeg = 1024;
-constexpr uint32_t kT1Slots = 2048;
-constexpr int kT1Waves = 4;
+constexpr uint32_t kT1MaxEdges = 256;
 constexpr double kT1UnitWork = 65536.0;
 
-constexpr uint32_t kT2SegCap = 16384;
-constexpr uint32_t kT2Slots = 32768;
-constexpr int kT2Threads = 1024;
 constexpr uint32_t kT2MaxEdges = 512;
 constexpr double kT2UnitWork = 1048576.0;
 
@@ -88,10 +84,11 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
   for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
     const uint64_t k = base + lane;
     bool owned = false;
-    uint32_t row = 0xFFFFFFFFu;
+    uint32_t row = 0xFFFFFFFFu;  // rows must not decrease across lanes for the segmented reduction
     uint64_t w = 0;
     if (k < m) {
       const uint32_t u = src[k];
+      row = u;
       const uint32_t v = (uint32_t)adj[k];
       const uint32_t du = deg_of(rp, u), dv = deg_of(rp, v);
       // orientation of GPUSCAN++/Graph.cpp:75-83: a = lower (degree, id)
@@ -109,23 +106,22 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       flag[k] = f;
       if (f == kFlagUDown) {
         owned = true;
-        row = u;
         w = dv;
       }
     }
     const uint64_t om = __ballot(owned);
     if (om) {
-      owned_total += (uint32_t)__popcll(om);
+      if (lane == 0) owned_total += (uint32_t)__popcll(om);
       const uint32_t c = wave_seg_reduce(row, owned ? 1u : 0u, OpAdd());
       const uint64_t ws = wave_seg_reduce(row, w, OpAdd());
       const bool head = wave_seg_head(row);  // all lanes: the shuffle reads the previous lane
-      if (owned && head) {
+      if (head && c) {
         atomicAdd(&cnt_u[row], c);
         atomicAdd((unsigned long long*)&work_u[row], (unsigned long long)ws);
       }
     }
   }
-  if (lane == 0 && owned_total) atomicAdd((unsigned long long*)&cnt[kCntUEdges], (unsigned long long)owned_total);
+  block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
         start = ustart[x];
         const uint32_t d = deg_of(rp, (uint32_t)x);
         if (d <= kT1MaxDeg) {
-          per = edges_per_unit(owned, work_u[x], kT1UnitWork, owned);
+          per = edges_per_unit(owned, work_u[x], kT1UnitWork, kT1MaxEdges);
           c1 = (owned + per - 1) / per;
         } else {
           per = edges_per_unit(owned, work_u[x], kT2UnitWork, kT2MaxEdges);
@@ -295,238 +291,290 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
 // 4. Counting
 // ---------------------------------------------------------------------------
 
-// Result of streaming part of one short row against the table.
-struct StreamState {
-  uint32_t cn;       // common neighbours found so far
-  uint32_t pos;      // next unstreamed index of the short row
-  uint32_t mirror;   // index of b in the short row, kNone if not seen yet
-  bool decided;
-  bool sim;
+// Ids per bitmap window of the large tier: 2^20 bits = 128 KB of LDS.
+constexpr uint32_t kWindowBits = 20;
+
+// Tier shapes: threads per workgroup, LDS table kind (bitmap over id windows
+// or linear-probing hash table) and size in words, owned edges per unit,
+// elements per claim at most (CHUNK / 64 loads in flight per lane).
+template <int THREADS, bool BITMAP, uint32_t TABLE_WORDS, uint32_t MAX_EDGES, uint32_t CHUNK>
+struct Tier {
+  static constexpr int kThreads = THREADS;
+  static constexpr bool kBitmap = BITMAP;
+  static constexpr uint32_t kTableWords = TABLE_WORDS;
+  static constexpr uint32_t kMaxEdges = MAX_EDGES;
+  static constexpr uint32_t kChunk = CHUNK;
+  static constexpr uint32_t kLoads = CHUNK / kWave;
 };
+// Small tier: owner rows of at most 1024 ids in a hash table kept at most a
+// quarter full.  Large tier: one window of the owner row at a time as an
+// exact bitmap.
+using SmallTier = Tier<256, false, 4 * kT1MaxDeg, kT1MaxEdges, 256>;
+using LargeTier = Tier<1024, true, (1u << kWindowBits) / 32, kT2MaxEdges, 1024>;
 
-// Streams adj[pos, hi) of the short row (which ends at ae) against the table,
-// 256 elements per step, checking the exact threshold after each step.
-// Wave-uniform arguments; returns the updated state.
-__device__ __forceinline__ StreamState stream_probe(const int32_t* __restrict__ adj, const uint32_t* tab,
-                                                    uint32_t mask, uint32_t shift, uint32_t hi, uint32_t ae,
-                                                    int32_t b, bool thresholded, uint32_t cstar, StreamState st) {
-  const uint32_t lane = lane_id();
-  uint32_t my_mirror = kNone;
-  while (!st.decided && st.pos < hi) {
-    const uint32_t i0 = st.pos + lane;
-    const uint32_t i1 = i0 + 64, i2 = i0 + 128, i3 = i0 + 192;
-    const int32_t x0 = i0 < hi ? adj[i0] : -1;
-    const int32_t x1 = i1 < hi ? adj[i1] : -1;
-    const int32_t x2 = i2 < hi ? adj[i2] : -1;
-    const int32_t x3 = i3 < hi ? adj[i3] : -1;
-    const bool h0 = x0 >= 0 && table_contains(tab, mask, shift, (uint32_t)x0);
-    const bool h1 = x1 >= 0 && table_contains(tab, mask, shift, (uint32_t)x1);
-    const bool h2 = x2 >= 0 && table_contains(tab, mask, shift, (uint32_t)x2);
-    const bool h3 = x3 >= 0 && table_contains(tab, mask, shift, (uint32_t)x3);
-    if (x0 == b) my_mirror = i0;
-    if (x1 == b) my_mirror = i1;
-    if (x2 == b) my_mirror = i2;
-    if (x3 == b) my_mirror = i3;
-    st.cn += (uint32_t)(__popcll(__ballot(h0)) + __popcll(__ballot(h1)) + __popcll(__ballot(h2)) +
-                        __popcll(__ballot(h3)));
-    st.pos = st.pos + 256 < hi ? st.pos + 256 : hi;
-    if (thresholded) {
-      const uint32_t rest = ae - st.pos;
-      if (st.cn >= cstar) {
-        st.decided = true;
-        st.sim = true;
-      } else if (st.cn + rest < cstar) {
-        st.decided = true;
-        st.sim = false;
-      }
+// Elements to claim next for an edge in state (hits, consumed): the fewest
+// after which a decision can be possible, min(c* - hits, hits + rest - c* + 1),
+// rounded up to whole wave loads and capped at one chunk.  Rows counted fully
+// (c* = kNone) claim whole chunks.
+__device__ __forceinline__ uint32_t claim_size(unsigned long long st, uint32_t cstar, uint32_t s, uint32_t chunk) {
+  if (cstar == kNone) return chunk;
+  const uint32_t hits = (uint32_t)(st >> 32), used = (uint32_t)st;
+  const uint32_t rest = s - used;
+  const uint32_t to_reach = cstar - hits;              // hits < c* while undecided
+  const uint32_t to_fail = hits + rest - cstar + 1;    // hits + rest >= c* while undecided
+  uint32_t want = to_reach < to_fail ? to_reach : to_fail;
+  want = (want + kWave - 1) / kWave * kWave;
+  return want < chunk ? want : chunk;
+}
+
+// (hits << 32 | consumed) of an edge with threshold c* (kNone: count fully)
+// and short-row length s: decided once hits >= c* or hits + (s - consumed)
+// < c*.  hits only grows and consumed only grows, so any snapshot is safe.
+__device__ __forceinline__ bool edge_decided(unsigned long long st, uint32_t cstar, uint32_t s) {
+  if (cstar == kNone) return false;
+  const uint32_t hits = (uint32_t)(st >> 32), used = (uint32_t)st;
+  return hits >= cstar || hits + (s - used) < cstar;
+}
+
+// Hash probes of x[0..N) (negative = no key), all N linear probes advancing
+// together; each round issues its N LDS reads before looking at any, so a
+// round costs one LDS latency however many probes are still running.
+template <uint32_t N>
+__device__ __forceinline__ uint32_t probe_hash(const uint32_t* tab, uint32_t mask, uint32_t shift, const int32_t* x) {
+  uint32_t slot[N];
+  uint32_t pend = 0;  // bit j: probe j still running
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) {
+    slot[j] = hash_slot((uint32_t)x[j], shift);
+    if (x[j] >= 0) pend |= 1u << j;
+  }
+  uint32_t h = 0;
+  while (__ballot(pend != 0)) {
+    uint32_t v[N];
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) v[j] = tab[slot[j]];
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      const bool on = (pend >> j) & 1u;
+      const bool found = on && v[j] == (uint32_t)x[j];
+      const bool stop = on && (found || v[j] == kEmptySlot);
+      h += found ? 1u : 0u;
+      if (stop) pend &= ~(1u << j);
+      slot[j] = (slot[j] + 1) & mask;
     }
   }
-  const uint64_t mm = __ballot(my_mirror != kNone);
-  if (mm) st.mirror = __shfl(my_mirror, __ffsll((long long)mm) - 1);
-  return st;
+  return h;
 }
 
-// Writes the decision to the owned entry k and to the mirror entry in the
-// short row [ab, ae) (searching the part not streamed when it was not seen).
-__device__ __forceinline__ void write_result(const int32_t* __restrict__ adj, uint8_t* __restrict__ flag, uint32_t k,
-                                             int32_t b, uint32_t ab, uint32_t ae, const StreamState& st) {
-  uint32_t mpos = st.mirror;
-  if (mpos == kNone) {
-    const uint32_t p = wave_lower_bound(adj, ab, ae, b);
-    mpos = p < ae && adj[p] == b ? p : kNone;
-  }
-  if (lane_id() == 0) {
-    const uint8_t f = st.sim ? kFlagY : kFlagN;
-    flag[k] = f;
-    if (mpos != kNone) flag[mpos] = f;
-  }
-}
-
-__device__ __forceinline__ StreamState initial_state(bool thresholded, uint32_t cstar, uint32_t s, uint32_t ab) {
-  StreamState st;
-  st.cn = 0;
-  st.pos = ab;
-  st.mirror = kNone;
-  st.decided = false;
-  st.sim = false;
-  if (thresholded && cstar == 0) {
-    st.decided = true;
-    st.sim = true;
-  } else if (thresholded && cstar > s) {
-    st.decided = true;
-  }
-  return st;
-}
-
-// Wave tier: one wave per unit, private table.
-__global__ void __launch_bounds__(kT1Waves * kWave) k_sim_wave_tier(uint32_t m, const uint32_t* __restrict__ rp,
-                                                                    const int32_t* __restrict__ adj,
-                                                                    const uint32_t* __restrict__ src, double eps,
-                                                                    uint8_t* __restrict__ flag,
-                                                                    const uint32_t* __restrict__ ulist,
-                                                                    const uint2* __restrict__ units,
-                                                                    uint64_t* __restrict__ cnt) {
-  __shared__ uint32_t tabs[kT1Waves][kT1Slots];
+// One unit = one owner b and up to MAX_EDGES of its undecided edges (a, b).
+// The owner row N(b) goes into LDS (small tier: hash table; large tier: one
+// 2^20-id window at a time as an exact bitmap).  The short rows N(a) are
+// streamed in claims that every wave of the workgroup takes dynamically: a
+// wave walks one edge in order, claiming each time just enough elements for a
+// decision to become possible, takes the next unstarted edge when its edge
+// closes, and helps unfinished edges once all are started.  Per edge,
+// (hits, consumed) live in one 64-bit LDS word updated by a single atomic
+// add, so every read is a consistent snapshot for the early exit.  In the
+// large tier a short row's slice for the current window ends where its
+// values reach the window's end; the claim that crosses it records the
+// boundary, so no search is needed.  Rows too long for the threshold form
+// (s > kMaxThresholdRow) are counted fully.
+template <class T>
+__global__ void __launch_bounds__(T::kThreads)
+    k_sim_units(uint32_t m, const uint32_t* __restrict__ rp, const int32_t* __restrict__ adj,
+                const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
+                const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
+                int units_slot, int queue_slot) {
+  __shared__ uint32_t tab[T::kTableWords];
+  __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
+  __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
+  __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
+  __shared__ uint32_t s_unit, s_cursor, s_kend;
   const uint32_t lane = lane_id();
-  uint32_t* tab = tabs[threadIdx.x / kWave];
-  const uint64_t nunits = cnt[kCntUnits1];
+  const uint64_t nunits = cnt[units_slot];
   uint64_t streamed = 0;
   for (;;) {
-    uint64_t u = 0;
-    if (lane == 0) u = atomicAdd((unsigned long long*)&cnt[kCntQueue1], 1ull);
-    u = __shfl(u, 0);
-    if (u >= nunits) break;
-    const uint2 un = units[u];
-    const uint32_t k0 = ulist[un.x];
-    if (k0 >= m) continue;  // never for consistent counts; keeps every index in range
-    const uint32_t b = src[k0];
-    const uint32_t bb = rp[b], be = rp[b + 1], l = be - bb;
-    if (l > kT1MaxDeg) continue;  // the planner sends only short owner rows here
-    const uint32_t T = table_size_for(l);
-    const uint32_t mask = T - 1, shift = 32 - log2_pow2(T);
-    for (uint32_t t = lane; t < T; t += kWave) tab[t] = kEmptySlot;
-    wave_lds_sync();
-    for (uint32_t t = bb + lane; t < be; t += kWave) table_insert(tab, mask, shift, (uint32_t)adj[t]);
-    wave_lds_sync();
-    for (uint32_t j0 = un.x; j0 < un.y; j0 += kWave) {
-      // lane t fetches edge j0 + t
-      uint32_t my_k = kNone, my_ab = 0, my_ae = 0;
-      if (j0 + lane < un.y) {
-        my_k = ulist[j0 + lane];
-        if (my_k < m) {
-          const uint32_t a = (uint32_t)adj[my_k];
-          my_ab = rp[a];
-          my_ae = rp[a + 1];
-        }
-      }
-      const uint32_t cnt_here = un.y - j0 < (uint32_t)kWave ? un.y - j0 : (uint32_t)kWave;
-      for (uint32_t e = 0; e < cnt_here; ++e) {
-        const uint32_t k = __shfl(my_k, e), ab = __shfl(my_ab, e), ae = __shfl(my_ae, e);
-        if (k >= m) continue;
-        const uint32_t s = ae - ab;
-        const double rhs = predicate_rhs(eps, s, l);
-        const bool thresholded = s <= kMaxThresholdRow;
-        const uint32_t cstar = thresholded ? min_common_threshold(rhs, s) : 0u;
-        StreamState st = initial_state(thresholded, cstar, s, ab);
-        st = stream_probe(adj, tab, mask, shift, ae, ae, (int32_t)b, thresholded, cstar, st);
-        if (!st.decided) st.sim = thresholded ? st.cn >= cstar : predicate_exact(st.cn, rhs);
-        streamed += st.pos - ab;
-        write_result(adj, flag, k, (int32_t)b, ab, ae, st);
-      }
-    }
-    wave_lds_sync();
-  }
-  if (lane == 0 && streamed) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
-}
-
-// Workgroup tier: 1024 threads per unit, shared table, value segments of the
-// owner's row for rows longer than kT2SegCap.
-__global__ void __launch_bounds__(kT2Threads) k_sim_group_tier(uint32_t m, const uint32_t* __restrict__ rp,
-                                                               const int32_t* __restrict__ adj,
-                                                               const uint32_t* __restrict__ src, double eps,
-                                                               uint8_t* __restrict__ flag,
-                                                               const uint32_t* __restrict__ ulist,
-                                                               const uint2* __restrict__ units,
-                                                               uint64_t* __restrict__ cnt) {
-  __shared__ uint32_t tab[kT2Slots];
-  __shared__ uint32_t e_k[kT2MaxEdges], e_cur[kT2MaxEdges], e_cn[kT2MaxEdges], e_mir[kT2MaxEdges];
-  __shared__ uint32_t s_unit, s_next;
-  const uint32_t lane = lane_id();
-  const uint64_t nunits = cnt[kCntUnits2];
-  uint64_t streamed = 0;
-  for (;;) {
-    if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[kCntQueue2], 1ull);
+    if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
     __syncthreads();
     const uint64_t u = s_unit;
     if (u >= nunits) break;
     const uint2 un = units[u];
     const uint32_t ne = un.y - un.x;
     const uint32_t k0 = ulist[un.x];
-    if (k0 >= m || ne > kT2MaxEdges) {  // never for consistent counts
-      __syncthreads();  // every thread has read s_unit before it is rewritten
+    const uint32_t b = k0 < m ? src[k0] : 0u;
+    const uint32_t bb = rp[b], be = rp[b + 1], l = be - bb;
+    if (k0 >= m || ne > T::kMaxEdges || ne == 0 || l == 0 || (!T::kBitmap && 4 * l > T::kTableWords)) {
+      __syncthreads();  // never for consistent plans; every thread has read s_unit
       continue;
     }
-    const uint32_t b = src[k0];
-    const uint32_t bb = rp[b], be = rp[b + 1], l = be - bb;
-    for (uint32_t t = threadIdx.x; t < ne; t += kT2Threads) {
+    if (threadIdx.x < ne) {
+      const uint32_t t = threadIdx.x;
       const uint32_t k = ulist[un.x + t];
-      const bool ok = k < m;
-      e_k[t] = k;
-      e_cur[t] = ok ? rp[(uint32_t)adj[k]] : 0u;
-      e_cn[t] = ok ? 0u : kNone;
-      e_mir[t] = kNone;
-    }
-    const uint32_t nseg = (l + kT2SegCap - 1) / kT2SegCap;
-    for (uint32_t seg = 0; seg < nseg; ++seg) {
-      const uint32_t sb = bb + seg * kT2SegCap;
-      const uint32_t se = sb + kT2SegCap < be ? sb + kT2SegCap : be;
-      const bool last = seg + 1 == nseg;
-      const int32_t vhi = last ? 0x7FFFFFFF : adj[se];  // this segment owns values < vhi
-      const uint32_t T = table_size_for(se - sb);
-      const uint32_t mask = T - 1, shift = 32 - log2_pow2(T);
-      for (uint32_t t = threadIdx.x; t < T; t += kT2Threads) tab[t] = kEmptySlot;
-      __syncthreads();
-      for (uint32_t t = sb + threadIdx.x; t < se; t += kT2Threads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
-      if (threadIdx.x == 0) s_next = 0;
-      __syncthreads();
-      for (;;) {
-        uint32_t e = 0;
-        if (lane == 0) e = atomicAdd(&s_next, 1u);
-        e = __shfl(e, 0);
-        if (e >= ne) break;
-        const uint32_t cn0 = e_cn[e];
-        if (cn0 == kNone) continue;  // decided in an earlier segment
-        const uint32_t k = e_k[e];
+      uint32_t ab = 0, s = 0, cstar = 0;
+      if (k < m) {
         const uint32_t a = (uint32_t)adj[k];
-        const uint32_t ab = rp[a], ae = rp[a + 1], s = ae - ab;
+        ab = rp[a];
+        s = rp[a + 1] - ab;
         const double rhs = predicate_rhs(eps, s, l);
-        const bool thresholded = s <= kMaxThresholdRow;
-        const uint32_t cstar = thresholded ? min_common_threshold(rhs, s) : 0u;
-        StreamState st = initial_state(thresholded, cstar, s, ab);
-        if (!st.decided) {
-          st.cn = cn0;
-          st.pos = e_cur[e];
-          st.mirror = e_mir[e];
-          const uint32_t hi = last ? ae : wave_lower_bound(adj, st.pos, ae, vhi);
-          const uint32_t p0 = st.pos;
-          st = stream_probe(adj, tab, mask, shift, hi, ae, (int32_t)b, thresholded, cstar, st);
-          streamed += st.pos - p0;
-          if (!st.decided && last) {
-            st.decided = true;
-            st.sim = thresholded ? st.cn >= cstar : predicate_exact(st.cn, rhs);
+        cstar = s <= kMaxThresholdRow ? min_common_threshold(rhs, s) : kNone;
+      }
+      e_k[t] = k;
+      e_ab[t] = ab;
+      e_s[t] = s;
+      e_cur[t] = ab;
+      e_cstar[t] = cstar;
+      e_mir[t] = kNone;
+      e_state[t] = 0ull;
+    }
+    uint32_t kcur = bb;
+    while (kcur < be) {
+      // ---- LDS table for owner keys adj[kcur, kend)
+      uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0;
+      if (T::kBitmap) {
+        const uint32_t w = (uint32_t)adj[kcur] >> kWindowBits;
+        wlo = w << kWindowBits;
+        whi = wlo + (1u << kWindowBits);
+        if (threadIdx.x < kWave) {
+          const uint32_t e = whi > 0x7FFFFFFFu ? be : wave_lower_bound(adj, kcur, be, (int32_t)whi);
+          if (lane == 0) s_kend = e;
+        }
+        for (uint32_t t = threadIdx.x; t < T::kTableWords; t += T::kThreads) tab[t] = 0u;
+        __syncthreads();
+        kend = s_kend;
+        for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) {
+          const uint32_t x = (uint32_t)adj[t] - wlo;
+          atomicOr(&tab[x >> 5], 1u << (x & 31));
+        }
+      } else {
+        const uint32_t TS = table_size_for(2 * l);  // at most a quarter full
+        mask = TS - 1;
+        shift = 32 - log2_pow2(TS);
+        for (uint32_t t = threadIdx.x; t < TS; t += T::kThreads) tab[t] = kEmptySlot;
+        __syncthreads();
+        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
+      }
+      const bool last = kend >= be;
+      if (threadIdx.x < ne) {
+        e_pos[threadIdx.x] = e_cur[threadIdx.x];
+        e_hi[threadIdx.x] = e_ab[threadIdx.x] + e_s[threadIdx.x];
+      }
+      if (threadIdx.x == 0) s_cursor = 0;
+      __syncthreads();  // table and per-edge cursors ready
+      // ---- stream the short rows
+      uint32_t cur_e = kNone;
+      for (;;) {
+        uint32_t e = kNone, start = 0, end = 0;
+        if (cur_e != kNone) {
+          const unsigned long long st = e_state[cur_e];
+          const uint32_t cstar = e_cstar[cur_e], s_e = e_s[cur_e];
+          uint32_t got = 0xFFFFFFFFu;
+          if (!edge_decided(st, cstar, s_e)) {
+            const uint32_t want = claim_size(st, cstar, s_e, T::kChunk);
+            if (lane == 0) got = atomicAdd(&e_pos[cur_e], want);
+            got = __shfl(got, 0);
+            const uint32_t ehi = e_hi[cur_e];
+            if (got < ehi) {
+              e = cur_e;
+              start = got;
+              end = got + want < ehi ? got + want : ehi;
+            }
           }
+          if (e == kNone) cur_e = kNone;
         }
-        if (st.decided) {
-          write_result(adj, flag, k, (int32_t)b, ab, ae, st);
-          if (lane == 0) e_cn[e] = kNone;
-        } else if (lane == 0) {
-          e_cn[e] = st.cn;
-          e_cur[e] = st.pos;
-          e_mir[e] = st.mirror;
+        if (e == kNone) {
+          uint32_t t = 0;
+          if (lane == 0) t = atomicAdd(&s_cursor, 1u);
+          t = __shfl(t, 0);
+          if (t < ne) {
+            cur_e = t;
+            continue;
+          }
+          uint32_t found = kNone;
+          for (uint32_t base = 0; base < ne && found == kNone; base += kWave) {
+            const uint32_t q = base + lane;
+            const bool open = q < ne && e_pos[q] < e_hi[q] && !edge_decided(e_state[q], e_cstar[q], e_s[q]);
+            const uint64_t bm = __ballot(open);
+            if (bm) found = base + (uint32_t)(__ffsll((long long)bm) - 1);
+          }
+          if (found == kNone) break;
+          cur_e = found;
+          continue;
         }
+        int32_t x[T::kLoads];
+#pragma unroll
+        for (uint32_t j = 0; j < T::kLoads; ++j) {
+          const uint32_t idx = start + lane + j * kWave;
+          x[j] = idx < end ? adj[idx] : -1;
+        }
+        uint32_t h = 0, used = end - start;
+        if (T::kBitmap) {
+          // first position whose value reaches the window's end (sorted row)
+          uint32_t bpos = end;
+          if (!last) {
+#pragma unroll
+            for (uint32_t j = 0; j < T::kLoads; ++j) {
+              const uint64_t over = __ballot(x[j] >= 0 && (uint32_t)x[j] >= whi);
+              if (over && bpos == end) bpos = start + j * kWave + (uint32_t)(__ffsll((long long)over) - 1);
+            }
+            if (bpos < end) {
+              used = bpos - start;
+              if (lane == 0) atomicMin(&e_hi[e], bpos);
+            }
+          }
+          uint32_t v[T::kLoads];
+#pragma unroll
+          for (uint32_t j = 0; j < T::kLoads; ++j) {
+            const uint32_t off = (uint32_t)x[j] - wlo;
+            v[j] = tab[(x[j] >= 0 && off < (1u << kWindowBits)) ? off >> 5 : 0u];
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < T::kLoads; ++j) {
+            const uint32_t off = (uint32_t)x[j] - wlo;
+            h += (x[j] >= 0 && off < (1u << kWindowBits)) ? (v[j] >> (off & 31)) & 1u : 0u;
+          }
+        } else {
+          h = probe_hash<T::kLoads>(tab, mask, shift, x);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < T::kLoads; ++j)
+          if (x[j] == (int32_t)b) e_mir[e] = start + lane + j * kWave;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) h += __shfl_xor(h, off);
+        if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
+        streamed += used;
       }
       __syncthreads();
+      if (threadIdx.x < ne) e_cur[threadIdx.x] = e_hi[threadIdx.x];
+      kcur = kend;
+      __syncthreads();
     }
+    // ---- decide and write both directed entries of every edge
+    if (threadIdx.x < ne) {
+      const uint32_t t = threadIdx.x;
+      const uint32_t k = e_k[t];
+      if (k < m) {
+        const uint32_t hits = (uint32_t)(e_state[t] >> 32);
+        const uint32_t cstar = e_cstar[t];
+        const uint32_t ab = e_ab[t], s = e_s[t];
+        const bool sim = cstar != kNone ? hits >= cstar : predicate_exact(hits, predicate_rhs(eps, s, l));
+        uint32_t mpos = e_mir[t];
+        if (mpos == kNone) {
+          uint32_t lo = ab, hi = ab + s;
+          while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (adj[mid] < (int32_t)b)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          mpos = lo < ab + s && adj[lo] == (int32_t)b ? lo : kNone;
+        }
+        const uint8_t f = sim ? kFlagY : kFlagN;
+        flag[k] = f;
+        if (mpos != kNone) flag[mpos] = f;
+      }
+    }
+    __syncthreads();
   }
   if (lane == 0 && streamed) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
 }
@@ -565,10 +613,10 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
-  k_sim_group_tier<<<cus, kT2Threads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.units2,
-                                              b.cnt);
-  k_sim_wave_tier<<<cus * 5, kT1Waves * kWave, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist,
-                                                       b.units1, b.cnt);
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
+                                                             b.ulist, b.units2, b.cnt, kCntUnits2, kCntQueue2);
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
+                                                                 b.ulist, b.units1, b.cnt, kCntUnits1, kCntQueue1);
 }
 
 }  // namespace sgc

@@ -176,6 +176,7 @@ typedef struct {
   uint64_t outliers;
   double alg_bytes;      /* algorithmic bytes of the similarity kernels */
   double probe_elements; /* list elements actually streamed by them */
+  double debug[10];      /* phase timers of instrumented builds; 0 otherwise */
 } sgc_stats;
 
 /* Replaces GPUScan(graph_t*, int*, int*, double, int) (GPUSCAN++/GPUScan.h:7,
