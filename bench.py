@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--eps", default="0.2,0.4,0.6,0.8")
     ap.add_argument("--mu", default="2,5,10")
     ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget; 0 skips it")
+    ap.add_argument("--copy-gib", type=float, default=1.0,
+                    help="size of the plain device copy run first as the HBM counter yardstick; 0 skips it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -109,6 +111,12 @@ def main() -> int:
     work = {eps: g.similarity_work(eps) for eps in eps_list}
 
     ctx = sgc.Context(0)
+    copy = None
+    if args.copy_gib > 0:
+        nbytes = int(args.copy_gib * (1 << 30))
+        ms = sgc.copy_bench(ctx, nbytes)
+        copy = {"bytes": nbytes, "ms": round(ms, 4), "GB/s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+        log(f"plain copy of {nbytes} bytes: {ms:.3f} ms = {copy['GB/s']} GB/s read+write")
     t0 = time.perf_counter()
     dg = sgc.DeviceGraph(ctx, g)
     log(f"device {ctx.device_name}: upload {time.perf_counter() - t0:.2f} s")
@@ -170,7 +178,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "similarity (k_similarity_wave)",
+            "kernel": "similarity (k_sim_units, both tiers)",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -181,6 +189,7 @@ def main() -> int:
             "avg_launch_ms": sim_ms / len(stats),
         },
         "cpu_baseline": cpu,
+        "copy_kernel": copy,
         "phases_ms_avg": {k: round(v, 3) for k, v in phases.items()},
     }
     print(json.dumps(out), flush=True)

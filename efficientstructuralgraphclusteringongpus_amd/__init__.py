@@ -155,6 +155,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_scan_device", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_GPUScan", _P(C.c_int32), [_P(_GraphT), _VP, _VP, C.c_double, C.c_int32])
+    _sig(lib, "sgc_copy_bench", C.c_int, [_VP, C.c_uint64, _P(C.c_double)])
     _LIB = lib
     return lib
 
@@ -379,6 +380,14 @@ def scan_device(ctx: Context, dg: DeviceGraph, eps: float, mu: int, d_labels: in
     _check(lib.sgc_scan_device(ctx.handle, dg.handle, float(eps), int(mu), d_labels, d_core, d_sim,
                                C.byref(st)))
     return ScanStats._from(st)
+
+
+def copy_bench(ctx: Context, nbytes: int) -> float:
+    """Milliseconds of one plain device copy of nbytes (HBM reference point)."""
+    lib = load_library()
+    ms = C.c_double()
+    _check(lib.sgc_copy_bench(ctx.handle, int(nbytes), C.byref(ms)))
+    return ms.value
 
 
 def GPUScan(folder: str, eps: float, miu: int) -> np.ndarray:

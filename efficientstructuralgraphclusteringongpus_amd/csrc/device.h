@@ -113,6 +113,7 @@ size_t scan_tmp_words(uint64_t nitems);
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
+void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);

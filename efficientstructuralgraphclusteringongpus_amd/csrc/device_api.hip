@@ -287,6 +287,33 @@ extern "C" int sgc_scan(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int
   return SGC_OK;
 }
 
+extern "C" int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms) {
+  clear_error();
+  if (!ctx || bytes < 16) return fail(SGC_EINVAL, "bad argument");
+  SGC_HIP(hipSetDevice(ctx->device));
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (a) (void)hipFree(a);
+    return fail(SGC_ENOMEM, "device allocation of 2 x %llu bytes failed", (unsigned long long)bytes);
+  }
+  hipError_t e = hipMemsetAsync(a, 1, bytes, ctx->stream);
+  if (e == hipSuccess) {
+    launch_copy(a, b, bytes, ctx->stream);  // warm
+    e = hipEventRecord(ctx->ev[0], ctx->stream);
+  }
+  if (e == hipSuccess) {
+    launch_copy(a, b, bytes, ctx->stream);
+    e = hipEventRecord(ctx->ev[1], ctx->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && ms) *ms = elapsed(ctx->ev[0], ctx->ev[1]);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(SGC_EDEVICE, "copy benchmark failed: %s", hipGetErrorString(e));
+  return SGC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Reference-compatible entry point
 // ---------------------------------------------------------------------------

@@ -17,6 +17,20 @@ __device__ __forceinline__ uint32_t deg_of(const uint32_t* rp, uint32_t x) { ret
 __device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane) { return (1ull << lane) - 1ull; }
 __device__ __forceinline__ uint64_t lanemask_le(uint32_t lane) { return lane == 63 ? ~0ull : (2ull << lane) - 1ull; }
 
+// Raw buffer resource over `bytes` bytes at p (gfx9 descriptor word 3): a
+// buffer load at or past the end returns 0 instead of touching memory, so a
+// partial wave load needs neither an exec mask nor a clamped address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// Lanes of load j (elements j*64 .. j*64+63) that lie inside a claim of len
+// elements, as a ballot mask.
+__device__ __forceinline__ uint64_t load_mask(uint32_t j, uint32_t len) {
+  const uint32_t r = len - j * kWave;
+  return r >= (uint32_t)kWave ? ~0ull : (1ull << r) - 1ull;
+}
+
 // Loads and stores that bypass the non-coherent per-CU cache: union-find reads
 // parents that other CUs are changing.
 __device__ __forceinline__ int32_t ld_coherent(const int32_t* p) {

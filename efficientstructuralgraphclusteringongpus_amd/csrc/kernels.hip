@@ -305,7 +305,19 @@ __global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint
 }
 
 
+// Plain streaming copy (16 bytes per lane per step), the reference point for
+// the HBM byte counters.
+__global__ void __launch_bounds__(kBlock) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / sizeof(uint4);
+  if (n) k_copy<<<grid_for(n, kBlock), kBlock, 0, s>>>((const uint4*)src, (uint4*)dst, n);
+}
 
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
   if (g->n == 0 || g->m == 0) return;

@@ -10,16 +10,16 @@
 // 3. k_plan: each owner's edges are cut into units of bounded work.
 //    Owners with d(b) <= kT1MaxDeg go to the wave tier, the rest to the
 //    workgroup tier.
-// 4. Counting: the owner's row N(b) (or one value segment of it) is put
-//    into an LDS hash table; the short rows N(a) of the unit's edges are cut
-//    into fixed-size items that every wave of the workgroup claims
-//    dynamically (16 coalesced loads per lane in flight for the large tier).
-//    Per edge the running (hits, consumed) pair decides early: reached ->
-//    similar, unreachable -> not similar (the reference's predicate in its
-//    exact threshold form).  The result is written to both directed entries.
-//    Small tier: owners of degree <= 1024, 256-thread workgroups.  Large
-//    tier: 1024-thread workgroups, 32768-slot table, rows longer than 16384
-//    split into value segments.
+// 4. Counting (k_sim_units): the owner's row N(b) is put into LDS and the
+//    short rows N(a) of the unit's edges are streamed through it in claims
+//    that every wave of the workgroup takes dynamically (up to 16 coalesced
+//    loads per lane in flight in the large tier).  Per edge the running
+//    (hits, consumed) pair decides early: reached -> similar, unreachable ->
+//    not similar (the reference's predicate in its exact threshold form).
+//    The result is written to both directed entries.
+//    Small tier: owners of degree <= 1024, 256-thread workgroups, the row as
+//    a linear-probing hash table.  Large tier: 1024-thread workgroups, the
+//    row one 2^20-id window at a time as an exact 128 KB bitmap.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -291,26 +291,26 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
 // 4. Counting
 // ---------------------------------------------------------------------------
 
-// Ids per bitmap window of the large tier: 2^20 bits = 128 KB of LDS.
+// Ids per bitmap window of the large tier: 2^20 bits = 128 KB of LDS, plus
+// one word that stays zero and absorbs every probe outside the window.
 constexpr uint32_t kWindowBits = 20;
+constexpr uint32_t kWindowWords = (1u << kWindowBits) / 32;
 
-// Tier shapes: threads per workgroup, LDS table kind (bitmap over id windows
-// or linear-probing hash table) and size in words, owned edges per unit,
-// elements per claim at most (CHUNK / 64 loads in flight per lane).
-template <int THREADS, bool BITMAP, uint32_t TABLE_WORDS, uint32_t MAX_EDGES, uint32_t CHUNK>
+// Tier shapes: threads per workgroup, LDS words for the owner-row table,
+// whether the table is a window bitmap (else a hash table of the whole row,
+// at most a quarter full), owned edges per unit, elements per claim at most
+// (CHUNK / 64 loads in flight per lane).
+template <int THREADS, uint32_t TABLE_WORDS, bool BITMAP, uint32_t MAX_EDGES, uint32_t CHUNK>
 struct Tier {
   static constexpr int kThreads = THREADS;
-  static constexpr bool kBitmap = BITMAP;
   static constexpr uint32_t kTableWords = TABLE_WORDS;
+  static constexpr bool kBitmap = BITMAP;
   static constexpr uint32_t kMaxEdges = MAX_EDGES;
   static constexpr uint32_t kChunk = CHUNK;
   static constexpr uint32_t kLoads = CHUNK / kWave;
 };
-// Small tier: owner rows of at most 1024 ids in a hash table kept at most a
-// quarter full.  Large tier: one window of the owner row at a time as an
-// exact bitmap.
-using SmallTier = Tier<256, false, 4 * kT1MaxDeg, kT1MaxEdges, 256>;
-using LargeTier = Tier<1024, true, (1u << kWindowBits) / 32, kT2MaxEdges, 1024>;
+using SmallTier = Tier<256, 4 * kT1MaxDeg, false, kT1MaxEdges, 256>;
+using LargeTier = Tier<1024, kWindowWords + 4, true, kT2MaxEdges, 1024>;
 
 // Elements to claim next for an edge in state (hits, consumed): the fewest
 // after which a decision can be possible, min(c* - hits, hits + rest - c* + 1),
@@ -336,17 +336,19 @@ __device__ __forceinline__ bool edge_decided(unsigned long long st, uint32_t cst
   return hits >= cstar || hits + (s - used) < cstar;
 }
 
-// Hash probes of x[0..N) (negative = no key), all N linear probes advancing
-// together; each round issues its N LDS reads before looking at any, so a
-// round costs one LDS latency however many probes are still running.
+// Hash probes of the wave's N loads x[j] (lanes inside a claim of len elements), all
+// linear probes advancing together; each round issues its LDS reads before
+// looking at any, so a round costs one LDS latency however many probes are
+// still running.  Returns the wave's number of keys found (uniform).
 template <uint32_t N>
-__device__ __forceinline__ uint32_t probe_hash(const uint32_t* tab, uint32_t mask, uint32_t shift, const int32_t* x) {
+__device__ __forceinline__ uint32_t probe_hash(const uint32_t* tab, uint32_t mask, uint32_t shift, const int32_t* x,
+                                               uint32_t len) {
   uint32_t slot[N];
   uint32_t pend = 0;  // bit j: probe j still running
 #pragma unroll
   for (uint32_t j = 0; j < N; ++j) {
     slot[j] = hash_slot((uint32_t)x[j], shift);
-    if (x[j] >= 0) pend |= 1u << j;
+    if (lane_id() + j * kWave < len) pend |= 1u << j;
   }
   uint32_t h = 0;
   while (__ballot(pend != 0)) {
@@ -358,12 +360,94 @@ __device__ __forceinline__ uint32_t probe_hash(const uint32_t* tab, uint32_t mas
       const bool on = (pend >> j) & 1u;
       const bool found = on && v[j] == (uint32_t)x[j];
       const bool stop = on && (found || v[j] == kEmptySlot);
-      h += found ? 1u : 0u;
+      h += (uint32_t)__popcll(__ballot(found));
       if (stop) pend &= ~(1u << j);
       slot[j] = (slot[j] + 1) & mask;
     }
   }
   return h;
+}
+
+// Position of key b among the wave's N loads x[j] of the claim [s0, s0 + len),
+// if there, into *mir (b's entry in the short row, the edge's mirror).
+template <uint32_t N>
+__device__ __forceinline__ void record_mirror(const int32_t* x, uint32_t len, uint32_t b, uint32_t s0, uint32_t* mir) {
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) {
+    const uint64_t mm = __ballot(x[j] == (int32_t)b) & load_mask(j, len);
+    if (mm && lane_id() == 0) *mir = s0 + j * kWave + (uint32_t)(__ffsll((long long)mm) - 1);
+  }
+}
+
+// What a claim needs to know about the unit's current table.
+struct ClaimTable {
+  const int32_t* adj;
+  const uint32_t* tab;
+  uint32_t wlo, whi;    // large tier: the window [wlo, whi)
+  uint32_t mask, shift;  // small tier: hash table shape
+  uint32_t b;            // the owner
+  bool last;             // no later window
+  bool mir_here;         // the window holds id b
+};
+
+// One claim of NL = ceil(len / 64) wave loads, straight-line so that all its
+// global loads, then all its LDS reads, are in flight together.  Buffer loads
+// past the claim's end read 0; those lanes are kept out of every count.
+// Returns the hits (uniform); *used = elements consumed (large tier: up to the
+// window's end, which is also lowered into *ehi when the claim crosses it).
+template <class T, uint32_t NL>
+__device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0, uint32_t len, uint32_t* mir,
+                                                uint32_t* ehi, uint32_t* used_out) {
+  const uint32_t lane = lane_id();
+  const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(c.adj + s0, len * 4u);
+  int32_t x[NL];
+#pragma unroll
+  for (uint32_t j = 0; j < NL; ++j) x[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + j * kWave) * 4u, 0, 0);
+  uint32_t h = 0, used = len;
+  if constexpr (T::kBitmap) {
+    // first position whose value reaches whi (sorted row): from there on the
+    // elements belong to later windows and probe the zero word
+    if (!c.last) {
+      const int32_t whs = (int32_t)c.whi;  // 0 < whi < 2^31 whenever a later window exists
+#pragma unroll
+      for (uint32_t j = 0; j < NL; ++j) {
+        const uint64_t over = __ballot(x[j] >= whs);  // padding (0) never
+        if (over && used == len) used = j * kWave + (uint32_t)(__ffsll((long long)over) - 1);
+      }
+      if (used < len && lane == 0) atomicMin(ehi, s0 + used);
+    }
+    if (c.mir_here) record_mirror<NL>(x, len, c.b, s0, mir);
+    uint32_t v[NL];
+#pragma unroll
+    for (uint32_t j = 0; j < NL; ++j) {
+      x[j] = (int32_t)((uint32_t)x[j] - c.wlo);  // offset: >= 2^20 outside the window
+      const uint32_t wi = (uint32_t)x[j] >> 5;
+      v[j] = c.tab[wi < kWindowWords ? wi : kWindowWords];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < NL; ++j) h += (uint32_t)__popcll(__ballot((v[j] >> ((uint32_t)x[j] & 31)) & 1u));
+    // lanes past the claim probed id 0: a hit only in window 0
+    const uint32_t pad = NL * kWave - len;
+    if (pad && c.wlo == 0 && (c.tab[0] & 1u)) h -= pad;
+  } else {
+    record_mirror<NL>(x, len, c.b, s0, mir);
+    h = probe_hash<NL>(c.tab, c.mask, c.shift, x, len);
+  }
+  *used_out = used;
+  return h;
+}
+
+// count_claim<T, nl> for a runtime nl in [LO, HI], by binary dispatch.
+template <class T, uint32_t LO, uint32_t HI>
+__device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable& c, uint32_t s0, uint32_t len,
+                                                   uint32_t* mir, uint32_t* ehi, uint32_t* used) {
+  if constexpr (LO == HI) {
+    return count_claim<T, LO>(c, s0, len, mir, ehi, used);
+  } else {
+    constexpr uint32_t MID = (LO + HI) / 2;
+    if (nl <= MID) return count_dispatch<T, LO, MID>(nl, c, s0, len, mir, ehi, used);
+    return count_dispatch<T, MID + 1, HI>(nl, c, s0, len, mir, ehi, used);
+  }
 }
 
 // One unit = one owner b and up to MAX_EDGES of its undecided edges (a, b).
@@ -377,7 +461,10 @@ __device__ __forceinline__ uint32_t probe_hash(const uint32_t* tab, uint32_t mas
 // add, so every read is a consistent snapshot for the early exit.  In the
 // large tier a short row's slice for the current window ends where its
 // values reach the window's end; the claim that crosses it records the
-// boundary, so no search is needed.  Rows too long for the threshold form
+// boundary, so no search is needed, and claims are capped near the slice
+// end predicted from the owner row's share of values below it, so little is
+// loaded past the boundary.  Load, probe and count instructions run only for
+// the claim's filled 64-element loads.  Rows too long for the threshold form
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
 __global__ void __launch_bounds__(T::kThreads)
@@ -385,14 +472,20 @@ __global__ void __launch_bounds__(T::kThreads)
                 const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
                 const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
                 int units_slot, int queue_slot) {
-  __shared__ uint32_t tab[T::kTableWords];
+  __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
   __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
   __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
+  __shared__ uint32_t e_cap[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend;
   const uint32_t lane = lane_id();
   const uint64_t nunits = cnt[units_slot];
   uint64_t streamed = 0;
+#ifdef SGC_PHASE_TIMERS
+  // debug builds: per-wave claims and wall-clock ticks (100 MHz) spent
+  // loading and probing them, summed into counter slots kCntTimer0 + 0..3
+  uint64_t dbg_claims = 0, dbg_load = 0, dbg_probe = 0, dbg_t = 0;
+#endif
   for (;;) {
     if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
     __syncthreads();
@@ -403,7 +496,7 @@ __global__ void __launch_bounds__(T::kThreads)
     const uint32_t k0 = ulist[un.x];
     const uint32_t b = k0 < m ? src[k0] : 0u;
     const uint32_t bb = rp[b], be = rp[b + 1], l = be - bb;
-    if (k0 >= m || ne > T::kMaxEdges || ne == 0 || l == 0 || (!T::kBitmap && 4 * l > T::kTableWords)) {
+    if (k0 >= m || ne > T::kMaxEdges || ne == 0 || l == 0) {
       __syncthreads();  // never for consistent plans; every thread has read s_unit
       continue;
     }
@@ -428,9 +521,11 @@ __global__ void __launch_bounds__(T::kThreads)
     }
     uint32_t kcur = bb;
     while (kcur < be) {
-      // ---- LDS table for owner keys adj[kcur, kend)
+      // ---- LDS table for owner keys adj[kcur, kend).  Large tier: exact
+      // bitmap of the 2^20-id window [wlo, whi) holding adj[kcur]; values >=
+      // whi belong to later windows.  Small tier: hash table of the whole row.
       uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0;
-      if (T::kBitmap) {
+      if constexpr (T::kBitmap) {
         const uint32_t w = (uint32_t)adj[kcur] >> kWindowBits;
         wlo = w << kWindowBits;
         whi = wlo + (1u << kWindowBits);
@@ -438,7 +533,8 @@ __global__ void __launch_bounds__(T::kThreads)
           const uint32_t e = whi > 0x7FFFFFFFu ? be : wave_lower_bound(adj, kcur, be, (int32_t)whi);
           if (lane == 0) s_kend = e;
         }
-        for (uint32_t t = threadIdx.x; t < T::kTableWords; t += T::kThreads) tab[t] = 0u;
+        uint4* tab4 = reinterpret_cast<uint4*>(tab);
+        for (uint32_t t = threadIdx.x; t < T::kTableWords / 4; t += T::kThreads) tab4[t] = make_uint4(0, 0, 0, 0);
         __syncthreads();
         kend = s_kend;
         for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) {
@@ -446,7 +542,7 @@ __global__ void __launch_bounds__(T::kThreads)
           atomicOr(&tab[x >> 5], 1u << (x & 31));
         }
       } else {
-        const uint32_t TS = table_size_for(2 * l);  // at most a quarter full
+        const uint32_t TS = table_size_for(2 * l);
         mask = TS - 1;
         shift = 32 - log2_pow2(TS);
         for (uint32_t t = threadIdx.x; t < TS; t += T::kThreads) tab[t] = kEmptySlot;
@@ -454,9 +550,23 @@ __global__ void __launch_bounds__(T::kThreads)
         for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
       }
       const bool last = kend >= be;
+      const bool mir_here = b - wlo < (1u << kWindowBits);  // large tier: the window holding id b
+      const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last, mir_here};
       if (threadIdx.x < ne) {
-        e_pos[threadIdx.x] = e_cur[threadIdx.x];
-        e_hi[threadIdx.x] = e_ab[threadIdx.x] + e_s[threadIdx.x];
+        const uint32_t t = threadIdx.x;
+        const uint32_t cur = e_cur[t], ab = e_ab[t], s = e_s[t];
+        e_pos[t] = cur;
+        e_hi[t] = ab + s;
+        uint32_t cap = T::kChunk;
+        if (!last) {
+          // the short row's slice is taken to end where the owner row's
+          // share of values below whi puts it
+          const uint32_t pe = ab + (uint32_t)((uint64_t)s * (kend - bb) / l);
+          const uint32_t pred = pe > cur ? pe - cur : 0u;
+          const uint32_t c = (pred + pred / 4 + 2 * kWave - 1) / kWave * kWave;
+          cap = c < T::kChunk ? c : T::kChunk;
+        }
+        e_cap[t] = cap;
       }
       if (threadIdx.x == 0) s_cursor = 0;
       __syncthreads();  // table and per-edge cursors ready
@@ -469,7 +579,7 @@ __global__ void __launch_bounds__(T::kThreads)
           const uint32_t cstar = e_cstar[cur_e], s_e = e_s[cur_e];
           uint32_t got = 0xFFFFFFFFu;
           if (!edge_decided(st, cstar, s_e)) {
-            const uint32_t want = claim_size(st, cstar, s_e, T::kChunk);
+            const uint32_t want = claim_size(st, cstar, s_e, e_cap[cur_e]);
             if (lane == 0) got = atomicAdd(&e_pos[cur_e], want);
             got = __shfl(got, 0);
             const uint32_t ehi = e_hi[cur_e];
@@ -500,46 +610,20 @@ __global__ void __launch_bounds__(T::kThreads)
           cur_e = found;
           continue;
         }
-        int32_t x[T::kLoads];
-#pragma unroll
-        for (uint32_t j = 0; j < T::kLoads; ++j) {
-          const uint32_t idx = start + lane + j * kWave;
-          x[j] = idx < end ? adj[idx] : -1;
-        }
-        uint32_t h = 0, used = end - start;
-        if (T::kBitmap) {
-          // first position whose value reaches the window's end (sorted row)
-          uint32_t bpos = end;
-          if (!last) {
-#pragma unroll
-            for (uint32_t j = 0; j < T::kLoads; ++j) {
-              const uint64_t over = __ballot(x[j] >= 0 && (uint32_t)x[j] >= whi);
-              if (over && bpos == end) bpos = start + j * kWave + (uint32_t)(__ffsll((long long)over) - 1);
-            }
-            if (bpos < end) {
-              used = bpos - start;
-              if (lane == 0) atomicMin(&e_hi[e], bpos);
-            }
-          }
-          uint32_t v[T::kLoads];
-#pragma unroll
-          for (uint32_t j = 0; j < T::kLoads; ++j) {
-            const uint32_t off = (uint32_t)x[j] - wlo;
-            v[j] = tab[(x[j] >= 0 && off < (1u << kWindowBits)) ? off >> 5 : 0u];
-          }
-#pragma unroll
-          for (uint32_t j = 0; j < T::kLoads; ++j) {
-            const uint32_t off = (uint32_t)x[j] - wlo;
-            h += (x[j] >= 0 && off < (1u << kWindowBits)) ? (v[j] >> (off & 31)) & 1u : 0u;
-          }
-        } else {
-          h = probe_hash<T::kLoads>(tab, mask, shift, x);
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < T::kLoads; ++j)
-          if (x[j] == (int32_t)b) e_mir[e] = start + lane + j * kWave;
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) h += __shfl_xor(h, off);
+#ifdef SGC_PHASE_TIMERS
+        dbg_t = wall_clock64();
+        ++dbg_claims;
+#endif
+        // the claim [s0, s0 + len) as wave-uniform values
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(start);
+        const uint32_t len = __builtin_amdgcn_readfirstlane(end - start);
+        uint32_t used;
+        const uint32_t h = count_dispatch<T, 1, T::kLoads>((len + kWave - 1) / kWave, cc, s0, len, &e_mir[e],
+                                                            &e_hi[e], &used);
+#ifdef SGC_PHASE_TIMERS
+        __builtin_amdgcn_s_waitcnt(0);
+        dbg_probe += wall_clock64() - dbg_t;
+#endif
         if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
         streamed += used;
       }
@@ -577,6 +661,15 @@ __global__ void __launch_bounds__(T::kThreads)
     __syncthreads();
   }
   if (lane == 0 && streamed) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
+#ifdef SGC_PHASE_TIMERS
+  if (lane == 0) {
+    const int base = kCntTimer0 + (T::kThreads == 1024 ? 0 : 4);
+    atomicAdd((unsigned long long*)&cnt[base + 0], (unsigned long long)dbg_claims);
+    atomicAdd((unsigned long long*)&cnt[base + 1], (unsigned long long)dbg_load);
+    atomicAdd((unsigned long long*)&cnt[base + 2], (unsigned long long)dbg_probe);
+    atomicAdd((unsigned long long*)&cnt[base + 3], (unsigned long long)streamed);
+  }
+#endif
 }
 
 }  // namespace

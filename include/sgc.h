@@ -194,6 +194,12 @@ int sgc_scan(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* labe
 int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* d_labels,
                     uint8_t* d_core, uint8_t* d_sim, sgc_stats* stats);
 
+/* Calibration aid with no counterpart in the reference: copies `bytes`
+ * between two fresh device buffers with a plain streaming kernel (k_copy),
+ * twice, and reports the second copy's time.  Its HBM byte counters
+ * (2 x bytes moved) are the yardstick for the counters of the SCAN kernels. */
+int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms);
+
 /* Reference-layout graph (GPUSCAN++/Graph.h:13-20). */
 typedef struct {
   long n;

@@ -120,6 +120,56 @@ def test_tiered_and_wave_kernels_agree(ctx, scale, ef, eps):
     assert np.array_equal(b.sim[sample], ref)
 
 
+def _kernels_agree(ctx, g, eps, mu, sample_size=200_000):
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        ctx.set_similarity_kernel(sgc.Context.SIM_WAVE_PER_EDGE)
+        a = sgc.scan(ctx, dg, eps, mu, want_sim=True)
+        ctx.set_similarity_kernel(sgc.Context.SIM_TIERED)
+        b = sgc.scan(ctx, dg, eps, mu, want_sim=True)
+    finally:
+        ctx.set_similarity_kernel(sgc.Context.SIM_TIERED)
+        dg.close()
+    assert np.array_equal(a.sim, b.sim)
+    assert np.array_equal(a.labels, b.labels)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    sample = np.random.default_rng(5).choice(len(adj), size=min(len(adj), sample_size), replace=False)
+    _, ref = ob.orc_similarity_entries(rp, adj, eps, sample.astype(np.uint64))
+    assert np.array_equal(b.sim[sample], ref)
+    return b
+
+
+@pytest.mark.parametrize("eps", [0.2, 0.6])
+def test_windows_beyond_one_bitmap(ctx, eps):
+    """n > 2^20: owner rows of the large tier span two 2^20-id bitmap windows;
+    each short row's slice per window is found while streaming."""
+    say("rmat21 windows", eps)
+    g = sgc.HostGraph.rmat(21, 16, seed=7)
+    assert g.n > (1 << 20) and g.max_degree > 4 * 8192
+    _kernels_agree(ctx, g, eps, 3)
+
+
+def test_bitmap_windows_for_very_long_owner_rows(ctx):
+    """n > 2^20 and owner rows of 140K keys spread over both 2^20-id windows,
+    with the mirror entries in either window."""
+    say("bitmap windows")
+    n = 1_100_000
+    k = 140_000
+    rng = np.random.default_rng(9)
+    common = np.arange(2, 2 + k, dtype=np.int64) * 7 % (n - 2) + 2  # spread over both windows
+    common = np.unique(common)
+    bg_src = rng.integers(0, n, size=2_000_000)
+    bg_dst = rng.integers(0, n, size=2_000_000)
+    src = np.concatenate([[0], np.zeros(len(common)), np.ones(len(common)), bg_src]).astype(np.int32)
+    dst = np.concatenate([[1], common, common, bg_dst]).astype(np.int32)
+    g = sgc.HostGraph.from_edges(n, src, dst)
+    assert g.max_degree > 131_072
+    for eps in (0.3, 0.9):
+        r = _kernels_agree(ctx, g, eps, 2, sample_size=400_000)
+    labels, core, sim, st = ob.orc_scan(np.array(g.row_ptr), np.array(g.adj), 0.9, 2)
+    assert np.array_equal(r.sim, sim) and np.array_equal(r.labels, labels)
+
+
 def test_planted_partition_hubs(ctx):
     """Hub-heavy graph: 64-vertex-scale version of the planted-partition
     workload (hub rows take the segmented workgroup tier)."""
