@@ -52,11 +52,12 @@ class _Stats(C.Structure):
     _fields_ = [
         ("ms_total", C.c_double), ("ms_prep", C.c_double), ("ms_similarity", C.c_double),
         ("ms_core", C.c_double), ("ms_cluster", C.c_double), ("ms_border", C.c_double),
-        ("ms_classify", C.c_double),
+        ("ms_classify", C.c_double), ("ms_similarity_large", C.c_double),
         ("edges", C.c_uint64), ("u_edges", C.c_uint64), ("similar_edges", C.c_uint64),
         ("cores", C.c_uint64), ("clusters", C.c_uint64), ("borders", C.c_uint64),
         ("hubs", C.c_uint64), ("outliers", C.c_uint64),
-        ("alg_bytes", C.c_double), ("probe_elements", C.c_double), ("debug", C.c_double * 10),
+        ("alg_bytes", C.c_double), ("probe_elements", C.c_double),
+        ("loaded_elements", C.c_double), ("claims", C.c_double),
     ]
 
 
@@ -74,6 +75,7 @@ class ScanStats:
     ms_cluster: float
     ms_border: float
     ms_classify: float
+    ms_similarity_large: float
     edges: int
     u_edges: int
     similar_edges: int
@@ -84,12 +86,12 @@ class ScanStats:
     outliers: int
     alg_bytes: float
     probe_elements: float
-    debug: tuple
+    loaded_elements: float
+    claims: float
 
     @classmethod
     def _from(cls, s: _Stats) -> "ScanStats":
         vals = {f.name: getattr(s, f.name) for f in fields(cls)}
-        vals["debug"] = tuple(s.debug)
         return cls(**vals)
 
 
@@ -140,7 +142,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_graph_digest", C.c_uint64, [_VP])
     _sig(lib, "sgc_graph_free", None, [_VP])
     _sig(lib, "sgc_graph_similarity_work", C.c_int,
-         [_VP, C.c_double, _P(C.c_uint64), _P(C.c_double), _P(C.c_double)])
+         [_VP, C.c_double, _P(C.c_uint64), _P(C.c_double), _P(C.c_double), _P(C.c_double)])
     _sig(lib, "sgc_philox4x32_10", None, [_VP, _VP, _VP])
     _sig(lib, "sgc_orient", C.c_int, [_VP, _VP, _VP, _VP])
     _sig(lib, "sgc_ctx_create", C.c_int, [C.c_int, _P(_VP)])
@@ -276,12 +278,15 @@ class HostGraph:
         return int(self._lib.sgc_graph_digest(self._h))
 
     def similarity_work(self, eps: float):
-        """(undecided edges, algorithmic bytes, short-row elements) at eps."""
+        """(undecided edges, algorithmic bytes, short-row elements, algorithmic
+        bytes of the edges whose owner has degree > 1024) at eps."""
         u = C.c_uint64()
         b = C.c_double()
         sh = C.c_double()
-        _check(self._lib.sgc_graph_similarity_work(self._h, float(eps), C.byref(u), C.byref(b), C.byref(sh)))
-        return int(u.value), float(b.value), float(sh.value)
+        bl = C.c_double()
+        _check(self._lib.sgc_graph_similarity_work(self._h, float(eps), C.byref(u), C.byref(b), C.byref(sh),
+                                                   C.byref(bl)))
+        return int(u.value), float(b.value), float(sh.value), float(bl.value)
 
     def orient(self):
         """(edge_u, edge_v, eid) as getEidAndEdgeList builds them."""

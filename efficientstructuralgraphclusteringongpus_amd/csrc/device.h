@@ -40,7 +40,8 @@ enum Counter {
   kCntQueue1,   // next wave-tier unit to claim
   kCntQueue2,   // next workgroup-tier unit to claim
   kCntQueueE,   // next edge for the one-wave-per-edge kernel
-  kCntTimer0 = 16,  // phase timers of debug builds (SGC_PHASE_TIMERS)
+  kCntLoaded,   // short-row elements loaded by the claims (consumed or not)
+  kCntClaims,   // claims taken
   kCntSlots = 32
 };
 
@@ -117,7 +118,8 @@ void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
+// `mid` (optional) is recorded between the workgroup-tier and wave-tier kernels.
+void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
                        hipStream_t s);
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
 void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);

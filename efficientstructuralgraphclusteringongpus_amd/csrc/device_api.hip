@@ -216,7 +216,7 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
   launch_prep(g, *pb, p, s);
   launch_plan(g, *pb, s);
   SGC_HIP(hipEventRecord(ctx->ev[1], s));
-  launch_similarity(g, *pb, p, ctx->cu_count, ctx->sim_impl, s);
+  launch_similarity(g, *pb, p, ctx->cu_count, ctx->sim_impl, ctx->ev[7], s);
   SGC_HIP(hipEventRecord(ctx->ev[2], s));
   launch_core(g, *pb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[3], s));
@@ -233,6 +233,7 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
     memset(st, 0, sizeof(*st));
     st->ms_prep = elapsed(ctx->ev[0], ctx->ev[1]);
     st->ms_similarity = elapsed(ctx->ev[1], ctx->ev[2]);
+    st->ms_similarity_large = g->m ? elapsed(ctx->ev[1], ctx->ev[7]) : 0.0;
     st->ms_core = elapsed(ctx->ev[2], ctx->ev[3]);
     st->ms_cluster = elapsed(ctx->ev[3], ctx->ev[4]);
     st->ms_border = elapsed(ctx->ev[4], ctx->ev[5]);
@@ -248,7 +249,8 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
     st->hubs = c[kCntHubs];
     st->outliers = c[kCntOutliers];
     st->probe_elements = (double)c[kCntProbes];
-    for (int i = 0; i < 10; ++i) st->debug[i] = (double)c[kCntTimer0 + i];
+    st->loaded_elements = (double)c[kCntLoaded];
+    st->claims = (double)c[kCntClaims];
   }
   return SGC_OK;
 }

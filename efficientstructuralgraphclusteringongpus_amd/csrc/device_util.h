@@ -27,6 +27,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uin
 // Lanes of load j (elements j*64 .. j*64+63) that lie inside a claim of len
 // elements, as a ballot mask.
 __device__ __forceinline__ uint64_t load_mask(uint32_t j, uint32_t len) {
+  if (j * kWave >= len) return 0ull;
   const uint32_t r = len - j * kWave;
   return r >= (uint32_t)kWave ? ~0ull : (1ull << r) - 1ull;
 }

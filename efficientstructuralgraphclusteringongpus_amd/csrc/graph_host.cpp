@@ -664,13 +664,13 @@ extern "C" int sgc_orient(const sgc_host_graph* g, int32_t* edge_u, int32_t* edg
 }
 
 extern "C" int sgc_graph_similarity_work(const sgc_host_graph* g, double eps, uint64_t* u_edges,
-                                         double* alg_bytes, double* short_elements) {
+                                         double* alg_bytes, double* short_elements, double* alg_bytes_large) {
   clear_error();
   if (!g) return fail(SGC_EINVAL, "null graph");
   uint64_t cnt = 0;
-  double bytes = 0, shorts = 0;
+  double bytes = 0, shorts = 0, large = 0;
   const uint64_t* rp = g->row_ptr;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : cnt, bytes, shorts)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : cnt, bytes, shorts, large)
   for (uint64_t u = 0; u < g->n; ++u) {
     const uint64_t du = rp[u + 1] - rp[u];
     for (uint64_t k = rp[u]; k < rp[u + 1]; ++k) {
@@ -681,13 +681,16 @@ extern "C" int sgc_graph_similarity_work(const sgc_host_graph* g, double eps, ui
       if (A < B * eps * eps || eps * eps * A * B <= 4.0) continue;
       ++cnt;
       double levels = std::ceil(std::log2((double)dv + 1.0));
-      bytes += 4.0 * ((double)du + std::min((double)dv, (double)du * levels));
+      const double eb = 4.0 * ((double)du + std::min((double)dv, (double)du * levels));
+      bytes += eb;
+      if (dv > 1024) large += eb;  // owner v in the workgroup tier
       shorts += (double)du;
     }
   }
   if (u_edges) *u_edges = cnt;
   if (alg_bytes) *alg_bytes = bytes;
   if (short_elements) *short_elements = shorts;
+  if (alg_bytes_large) *alg_bytes_large = large;
   return SGC_OK;
 }
 

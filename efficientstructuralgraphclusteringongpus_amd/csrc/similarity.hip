@@ -392,7 +392,8 @@ struct ClaimTable {
 
 // One claim of NL = ceil(len / 64) wave loads, straight-line so that all its
 // global loads, then all its LDS reads, are in flight together.  Buffer loads
-// past the claim's end read 0; those lanes are kept out of every count.
+// past the claim's end read 0 without touching memory; those lanes are kept
+// out of every count.
 // Returns the hits (uniform); *used = elements consumed (large tier: up to the
 // window's end, which is also lowered into *ehi when the claim crosses it).
 template <class T, uint32_t NL>
@@ -438,7 +439,7 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
 }
 
 // count_claim<T, nl> for a runtime nl in [LO, HI], by binary dispatch.
-template <class T, uint32_t LO, uint32_t HI>
+template <class T, uint32_t LO = 1, uint32_t HI = T::kLoads>
 __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable& c, uint32_t s0, uint32_t len,
                                                    uint32_t* mir, uint32_t* ehi, uint32_t* used) {
   if constexpr (LO == HI) {
@@ -480,12 +481,7 @@ __global__ void __launch_bounds__(T::kThreads)
   __shared__ uint32_t s_unit, s_cursor, s_kend;
   const uint32_t lane = lane_id();
   const uint64_t nunits = cnt[units_slot];
-  uint64_t streamed = 0;
-#ifdef SGC_PHASE_TIMERS
-  // debug builds: per-wave claims and wall-clock ticks (100 MHz) spent
-  // loading and probing them, summed into counter slots kCntTimer0 + 0..3
-  uint64_t dbg_claims = 0, dbg_load = 0, dbg_probe = 0, dbg_t = 0;
-#endif
+  uint64_t streamed = 0, loaded = 0, claims = 0;
   for (;;) {
     if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
     __syncthreads();
@@ -599,6 +595,7 @@ __global__ void __launch_bounds__(T::kThreads)
             cur_e = t;
             continue;
           }
+          // every edge started: help an unfinished one
           uint32_t found = kNone;
           for (uint32_t base = 0; base < ne && found == kNone; base += kWave) {
             const uint32_t q = base + lane;
@@ -610,22 +607,16 @@ __global__ void __launch_bounds__(T::kThreads)
           cur_e = found;
           continue;
         }
-#ifdef SGC_PHASE_TIMERS
-        dbg_t = wall_clock64();
-        ++dbg_claims;
-#endif
-        // the claim [s0, s0 + len) as wave-uniform values
+        // the claim [s0, s0 + len) as wave-uniform values; its loads are
+        // issued inside count_claim, sized to the claim
         const uint32_t s0 = __builtin_amdgcn_readfirstlane(start);
         const uint32_t len = __builtin_amdgcn_readfirstlane(end - start);
         uint32_t used;
-        const uint32_t h = count_dispatch<T, 1, T::kLoads>((len + kWave - 1) / kWave, cc, s0, len, &e_mir[e],
-                                                            &e_hi[e], &used);
-#ifdef SGC_PHASE_TIMERS
-        __builtin_amdgcn_s_waitcnt(0);
-        dbg_probe += wall_clock64() - dbg_t;
-#endif
+        const uint32_t h = count_dispatch<T>((len + kWave - 1) / kWave, cc, s0, len, &e_mir[e], &e_hi[e], &used);
         if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
         streamed += used;
+        loaded += len;
+        ++claims;
       }
       __syncthreads();
       if (threadIdx.x < ne) e_cur[threadIdx.x] = e_hi[threadIdx.x];
@@ -660,16 +651,11 @@ __global__ void __launch_bounds__(T::kThreads)
     }
     __syncthreads();
   }
-  if (lane == 0 && streamed) atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
-#ifdef SGC_PHASE_TIMERS
-  if (lane == 0) {
-    const int base = kCntTimer0 + (T::kThreads == 1024 ? 0 : 4);
-    atomicAdd((unsigned long long*)&cnt[base + 0], (unsigned long long)dbg_claims);
-    atomicAdd((unsigned long long*)&cnt[base + 1], (unsigned long long)dbg_load);
-    atomicAdd((unsigned long long*)&cnt[base + 2], (unsigned long long)dbg_probe);
-    atomicAdd((unsigned long long*)&cnt[base + 3], (unsigned long long)streamed);
+  if (lane == 0 && claims) {
+    atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
+    atomicAdd((unsigned long long*)&cnt[kCntLoaded], (unsigned long long)loaded);
+    atomicAdd((unsigned long long*)&cnt[kCntClaims], (unsigned long long)claims);
   }
-#endif
 }
 
 }  // namespace
@@ -698,16 +684,18 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
                                                    b.units2, b.cnt);
 }
 
-void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
+void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
                        hipStream_t s) {
   if (g->m == 0) return;
   if (impl == kSimWavePerEdge) {
     launch_similarity_wave(g, b, p, cu_count, s);
+    if (mid) (void)hipEventRecord(mid, s);
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
                                                              b.ulist, b.units2, b.cnt, kCntUnits2, kCntQueue2);
+  if (mid) (void)hipEventRecord(mid, s);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
                                                                  b.ulist, b.units1, b.cnt, kCntUnits1, kCntQueue1);
 }

@@ -115,9 +115,11 @@ int sgc_orient(const sgc_host_graph* g, int32_t* edge_u, int32_t* edge_v, uint32
  * edges the degree-only test leaves undecided and their algorithmic bytes
  * sum 4 * (s + min(l, s * ceil(log2(l + 1)))) over those edges (s <= l the
  * endpoint degrees): the short row streamed once, the long row streamed
- * once or probed once per binary-search level, whichever is smaller. */
+ * once or probed once per binary-search level, whichever is smaller.
+ * short_elements = sum of s; alg_bytes_large = the part of alg_bytes of
+ * edges whose higher endpoint has degree > 1024 (the workgroup tier). */
 int sgc_graph_similarity_work(const sgc_host_graph* g, double eps, uint64_t* u_edges, double* alg_bytes,
-                              double* short_elements);
+                              double* short_elements, double* alg_bytes_large);
 
 /* ------------------------------------------------------------------------ */
 /* Device context and device-resident graphs                                 */
@@ -166,6 +168,7 @@ typedef struct {
   double ms_cluster;    /* union-find + flatten */
   double ms_border;     /* border assignment */
   double ms_classify;   /* hub / outlier */
+  double ms_similarity_large; /* the part of ms_similarity spent on owners of degree > 1024 */
   uint64_t edges;       /* E = m/2 */
   uint64_t u_edges;     /* edges the degree-only test could not decide */
   uint64_t similar_edges;
@@ -175,8 +178,9 @@ typedef struct {
   uint64_t hubs;
   uint64_t outliers;
   double alg_bytes;      /* algorithmic bytes of the similarity kernels */
-  double probe_elements; /* list elements actually streamed by them */
-  double debug[10];      /* phase timers of instrumented builds; 0 otherwise */
+  double probe_elements; /* list elements consumed by them */
+  double loaded_elements; /* list elements loaded by them (consumed or read past a window's end) */
+  double claims;          /* wave claims of the tiered kernels */
 } sgc_stats;
 
 /* Replaces GPUScan(graph_t*, int*, int*, double, int) (GPUSCAN++/GPUScan.h:7,
