@@ -61,6 +61,24 @@ class _Stats(C.Structure):
     ]
 
 
+class _OocStats(C.Structure):
+    _fields_ = [
+        ("device_bytes", C.c_uint64), ("similarity_chips", C.c_uint32), ("row_chips", C.c_uint32),
+        ("bytes_h2d", C.c_double), ("bytes_d2h", C.c_double), ("ms_host", C.c_double), ("ms_phases", C.c_double),
+    ]
+
+
+@dataclass
+class OocStats:
+    device_bytes: int
+    similarity_chips: int
+    row_chips: int
+    bytes_h2d: float
+    bytes_d2h: float
+    ms_host: float
+    ms_phases: float
+
+
 class _GraphT(C.Structure):
     _fields_ = [("n", C.c_long), ("m", C.c_long), ("adj", C.POINTER(C.c_int32)),
                 ("num_edges", C.POINTER(C.c_uint32)), ("eid", C.POINTER(C.c_uint32))]
@@ -155,6 +173,9 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_dgraph_n", C.c_uint64, [_VP])
     _sig(lib, "sgc_dgraph_m", C.c_uint64, [_VP])
     _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
+    _sig(lib, "sgc_ooc_min_budget", C.c_int, [_VP, C.c_double, _P(C.c_uint64)])
+    _sig(lib, "sgc_scan_ooc", C.c_int,
+         [_VP, _VP, C.c_double, C.c_int32, C.c_uint64, _VP, _VP, _VP, _P(_Stats), _P(_OocStats)])
     _sig(lib, "sgc_scan_device", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_GPUScan", _P(C.c_int32), [_P(_GraphT), _VP, _VP, C.c_double, C.c_int32])
     _sig(lib, "sgc_copy_bench", C.c_int, [_VP, C.c_uint64, _P(C.c_double)])
@@ -374,6 +395,31 @@ def scan(ctx: Context, dg: DeviceGraph, eps: float, mu: int, want_sim: bool = Fa
     _check(lib.sgc_scan(ctx.handle, dg.handle, float(eps), int(mu), _ptr(labels), _ptr(core), _ptr(sim),
                         C.byref(st)))
     return ScanResult(labels[: dg.n], core[: dg.n], None if sim is None else sim[: dg.m], ScanStats._from(st))
+
+
+def ooc_min_budget(g: HostGraph, eps: float) -> int:
+    """Smallest device budget scan_ooc accepts for g at eps (bytes)."""
+    lib = load_library()
+    b = C.c_uint64()
+    _check(lib.sgc_ooc_min_budget(g._h, float(eps), C.byref(b)))
+    return int(b.value)
+
+
+def scan_ooc(ctx: Context, g: HostGraph, eps: float, mu: int, device_budget: int,
+             want_sim: bool = False) -> tuple[ScanResult, OocStats]:
+    """Out-of-core pass over a host-resident graph of any size within
+    `device_budget` bytes of device memory; results identical to scan()."""
+    lib = load_library()
+    n, m = g.n, g.m
+    labels = np.empty(max(n, 1), dtype=np.int32)
+    core = np.empty(max(n, 1), dtype=np.uint8)
+    sim = np.empty(max(m, 1), dtype=np.uint8) if want_sim else None
+    st = _Stats()
+    oi = _OocStats()
+    _check(lib.sgc_scan_ooc(ctx.handle, g._h, float(eps), int(mu), int(device_budget), _ptr(labels), _ptr(core),
+                            _ptr(sim), C.byref(st), C.byref(oi)))
+    info = OocStats(**{f.name: getattr(oi, f.name) for f in fields(OocStats)})
+    return ScanResult(labels[:n], core[:n], None if sim is None else sim[:m], ScanStats._from(st)), info
 
 
 def scan_device(ctx: Context, dg: DeviceGraph, eps: float, mu: int, d_labels: int | None = None,

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "common.h"
+#include "scan_rules.h"
 
 #define SGC_HIP(call)                                                                             \
   do {                                                                                            \
@@ -19,11 +20,6 @@ namespace sgc {
 constexpr int kWave = 64;
 constexpr int32_t kNoLabel = 0x7FFFFFFF;
 
-// Per-edge flag byte stored for every directed CSR entry.  Undecided edges are
-// marked in both rows: kFlagUDown in the higher (degree, id) endpoint's row,
-// which owns the computation, kFlagUUp in the other row, which receives the
-// result.  After the similarity phase every flag is kFlagN or kFlagY.
-enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagUUp = 2, kFlagUDown = 3 };
 
 // Counters written by the kernels (one 64-bit slot each).
 enum Counter {
@@ -76,6 +72,13 @@ struct sgc_dgraph {
   uint32_t* row_ptr = nullptr;  // n + 1
   int32_t* adj = nullptr;       // m
   uint32_t* src = nullptr;      // m: row of each entry
+  // Row x is adj[row_begin[x], row_end[x]) for every row the similarity
+  // kernels read; row_end[x] - row_begin[x] is x's degree for every x.  A
+  // resident graph uses row_ptr and row_ptr + 1; an out-of-core chip holds
+  // only some rows (absent rows: a range of the right length that is never
+  // read).
+  const uint32_t* row_begin = nullptr;
+  const uint32_t* row_end = nullptr;
 };
 
 namespace sgc {
@@ -126,5 +129,15 @@ void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStr
 void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+// The single kernels the four launchers above are made of (edge passes over
+// entries [0, g->m), vertex passes over all g->n vertices); the out-of-core
+// path runs the edge passes once per chip.
+void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 
 }  // namespace sgc

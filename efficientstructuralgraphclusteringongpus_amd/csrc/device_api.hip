@@ -174,6 +174,8 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
     return fail(SGC_ENOMEM, "device allocation for graph (n=%llu, m=%llu) failed", (unsigned long long)g->n,
                 (unsigned long long)g->m);
   }
+  d->row_begin = d->row_ptr;
+  d->row_end = d->row_ptr + 1;
   hipError_t e = hipMemcpyAsync(d->row_ptr, rp32.data(), (g->n + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess && g->m)
     e = hipMemcpyAsync(d->adj, g->adj, g->m * 4, hipMemcpyHostToDevice, ctx->stream);

@@ -58,7 +58,8 @@ __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restric
 // similar, unreachable -> not similar.  Writes the flag of both directed
 // entries.  Kept beside the tiered kernels as an independent second
 // implementation (selectable per context).
-__global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __restrict__ rp,
+__global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __restrict__ rb,
+                                                            const uint32_t* __restrict__ re,
                                                             const int32_t* __restrict__ adj,
                                                             const uint32_t* __restrict__ src, double eps,
                                                             uint8_t* __restrict__ flag,
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
     const uint32_t k = ulist[i];
     const uint32_t b = src[k];
     const uint32_t a = (uint32_t)adj[k];
-    const uint32_t ab = rp[a], ae = rp[a + 1], bb = rp[b], be = rp[b + 1];
+    const uint32_t ab = rb[a], ae = re[a], bb = rb[b], be = re[b];
     const uint32_t s = ae - ab, l = be - bb;
     const double rhs = predicate_rhs(eps, s, l);
     const bool thresholded = s <= kMaxThresholdRow;
@@ -331,17 +332,26 @@ void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
   if (g->m == 0) return;
   const uint32_t blocks = (uint32_t)(cu_count > 0 ? cu_count : 256) * 8;
-  k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_ptr, g->adj, g->src, p.eps, b.flag, b.ulist, b.cnt);
+  k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag, b.ulist,
+                                              b.cnt);
 }
 
-void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+// Edge passes take the entries [0, g->m) of g (a whole graph or one chip of
+// rows); vertex passes cover all g->n vertices.
+void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m) k_core_count<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.sd);
+}
+
+void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   const uint32_t thr = p.mu >= 1 ? (uint32_t)p.mu : 1u;
   k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, thr, b.core, b.parent, b.label);
 }
 
-void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m) k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent);
+}
+
+void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   k_flatten<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.core, b.parent, b.label);
 }
 
@@ -349,10 +359,32 @@ void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m) k_border<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.label);
 }
 
-void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m) k_classify<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.label, b.lo, b.hi);
+}
+
+void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, b.lo, b.hi, b.core, b.label, b.cnt);
+}
+
+void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m) k_count_similar<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, b.cnt);
+}
+
+void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  launch_core_count(g, b, s);
+  launch_core_mark(g, b, p, s);
+}
+
+void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+  launch_union(g, b, s);
+  launch_flatten(g, b, s);
+}
+
+void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  launch_classify_edges(g, b, s);
+  launch_finalize(g, b, p, s);
+  launch_count_similar(g, b, s);
 }
 
 }  // namespace sgc

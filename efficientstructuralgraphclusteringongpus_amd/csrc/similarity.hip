@@ -73,8 +73,9 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 // 1. Degree-only pre-decision (GPUSCAN++/GPUScan.cu:133-172)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rp,
-                                                 const int32_t* __restrict__ adj, const uint32_t* __restrict__ src,
+__global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
+                                                 const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
+                                                 const uint32_t* __restrict__ src,
                                                  double eps, uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
                                                  uint64_t* __restrict__ cnt) {
@@ -90,19 +91,8 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       const uint32_t u = src[k];
       row = u;
       const uint32_t v = (uint32_t)adj[k];
-      const uint32_t du = deg_of(rp, u), dv = deg_of(rp, v);
-      // orientation of GPUSCAN++/Graph.cpp:75-83: a = lower (degree, id)
-      const bool up = dv > du || (dv == du && v > u);
-      const uint32_t da = up ? du : dv, db = up ? dv : du;
-      const double A = (double)(da + 1u);
-      const double B = (double)(db + 1u);
-      uint8_t f;
-      if (A < B * eps * eps)
-        f = kFlagN;
-      else if (eps * eps * A * B <= 4.0)
-        f = kFlagY;
-      else
-        f = up ? kFlagUUp : kFlagUDown;
+      const uint32_t dv = re[v] - rb[v];
+      const uint8_t f = prep_flag(u, v, re[u] - rb[u], dv, eps);
       flag[k] = f;
       if (f == kFlagUDown) {
         owned = true;
@@ -243,7 +233,8 @@ __device__ __forceinline__ uint32_t edges_per_unit(uint32_t cnt, uint64_t work, 
   return e ? e : 1u;
 }
 
-__global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __restrict__ rp,
+__global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __restrict__ rb,
+                                                 const uint32_t* __restrict__ re,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
                                                  const uint32_t* __restrict__ ustart, uint2* __restrict__ units1,
@@ -257,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       owned = cnt_u[x];
       if (owned) {
         start = ustart[x];
-        const uint32_t d = deg_of(rp, (uint32_t)x);
+        const uint32_t d = re[x] - rb[x];
         if (d <= kT1MaxDeg) {
           per = edges_per_unit(owned, work_u[x], kT1UnitWork, kT1MaxEdges);
           c1 = (owned + per - 1) / per;
@@ -469,7 +460,8 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
 __global__ void __launch_bounds__(T::kThreads)
-    k_sim_units(uint32_t m, const uint32_t* __restrict__ rp, const int32_t* __restrict__ adj,
+    k_sim_units(uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
                 const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
                 int units_slot, int queue_slot) {
@@ -491,7 +483,7 @@ __global__ void __launch_bounds__(T::kThreads)
     const uint32_t ne = un.y - un.x;
     const uint32_t k0 = ulist[un.x];
     const uint32_t b = k0 < m ? src[k0] : 0u;
-    const uint32_t bb = rp[b], be = rp[b + 1], l = be - bb;
+    const uint32_t bb = rb[b], be = re[b], l = be - bb;
     if (k0 >= m || ne > T::kMaxEdges || ne == 0 || l == 0) {
       __syncthreads();  // never for consistent plans; every thread has read s_unit
       continue;
@@ -502,8 +494,8 @@ __global__ void __launch_bounds__(T::kThreads)
       uint32_t ab = 0, s = 0, cstar = 0;
       if (k < m) {
         const uint32_t a = (uint32_t)adj[k];
-        ab = rp[a];
-        s = rp[a + 1] - ab;
+        ab = rb[a];
+        s = re[a] - ab;
         const double rhs = predicate_rhs(eps, s, l);
         cstar = s <= kMaxThresholdRow ? min_common_threshold(rhs, s) : kNone;
       }
@@ -662,8 +654,8 @@ __global__ void __launch_bounds__(T::kThreads)
 
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag, b.cnt_u,
-                                                   b.work_u, b.cnt);
+  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag,
+                                                   b.cnt_u, b.work_u, b.cnt);
 }
 
 static void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t nitems, uint32_t* tmp, hipStream_t s) {
@@ -680,8 +672,8 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m == 0) return;
   exclusive_scan(b.cnt_u, b.ustart, g->n, b.scan_tmp, s);
   k_scatter<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.cnt_u, b.ustart, b.fill, b.ulist);
-  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_ptr, b.cnt_u, b.work_u, b.ustart, b.units1,
-                                                   b.units2, b.cnt);
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, b.cnt_u, b.work_u, b.ustart,
+                                                   b.units1, b.units2, b.cnt);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
@@ -693,10 +685,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
-  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
+                                                             b.flag,
                                                              b.ulist, b.units2, b.cnt, kCntUnits2, kCntQueue2);
   if (mid) (void)hipEventRecord(mid, s);
-  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_ptr, g->adj, g->src, p.eps, b.flag,
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src,
+                                                                 p.eps, b.flag,
                                                                  b.ulist, b.units1, b.cnt, kCntUnits1, kCntQueue1);
 }
 

@@ -204,6 +204,37 @@ int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_
  * (2 x bytes moved) are the yardstick for the counters of the SCAN kernels. */
 int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms);
 
+/* Out-of-core run details. */
+typedef struct {
+  uint64_t device_bytes;     /* device memory the run allocated (<= the budget) */
+  uint32_t similarity_chips; /* owner-row chips of the similarity phase */
+  uint32_t row_chips;        /* row chips of the core / cluster / border / classify passes */
+  double bytes_h2d;          /* host-to-device bytes */
+  double bytes_d2h;          /* device-to-host bytes */
+  double ms_host;            /* host time planning, assembling chips and merging flags */
+  double ms_phases;          /* wall time of the passes after similarity */
+} sgc_ooc_stats;
+
+/* Replaces the out-of-core GPUScan of GPUSCAN++O (SCANoffload.cu:922-1464,
+ * chips GPUSCAN++O/Graph.cpp:414-576) for a host-resident graph of any size
+ * (m may exceed 2^32; n < 2^31): at most `device_budget` bytes of device
+ * memory are allocated; the similarity phase runs over chips of owner rows
+ * plus the short rows their undecided edges read, the later phases over chips
+ * of rows, with the per-vertex state resident throughout.  Outputs as
+ * sgc_scan (host arrays, each optional) and bit-identical to it.
+ * stats->ms_* similarity fields are device time summed over chips; the
+ * per-phase fields are filled when all rows fit one chip; ms_total is wall
+ * time.  SGC_ENOMEM if the budget cannot hold the per-vertex state and the
+ * largest single row's chip. */
+int sgc_scan_ooc(sgc_ctx* ctx, const sgc_host_graph* g, double eps, int32_t mu, uint64_t device_budget,
+                 int32_t* labels, uint8_t* core, uint8_t* sim, sgc_stats* stats, sgc_ooc_stats* ooc);
+
+/* Smallest device budget sgc_scan_ooc accepts for g at eps: the per-vertex
+ * state plus the chip of the single most demanding owner row (its row and the
+ * short rows of its undecided edges).  No counterpart in the reference, whose
+ * chip planner takes a fixed budget and fails inside. */
+int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t* bytes);
+
 /* Reference-layout graph (GPUSCAN++/Graph.h:13-20). */
 typedef struct {
   long n;
