@@ -40,7 +40,7 @@ def library_path() -> str:
 class SGCError(RuntimeError):
     """A libsgc call failed; ``code`` is the SGC_E* status."""
 
-    CODES = {1: "EINVAL", 2: "EIO", 3: "EFORMAT", 4: "ENOMEM", 5: "EDEVICE", 6: "ERANGE"}
+    CODES = {1: "EINVAL", 2: "EIO", 3: "EFORMAT", 4: "ENOMEM", 5: "EDEVICE", 6: "ERANGE", 7: "EBUSY"}
 
     def __init__(self, code: int, message: str):
         super().__init__(f"[{self.CODES.get(code, code)}] {message}")
@@ -77,6 +77,29 @@ class OocStats:
     bytes_d2h: float
     ms_host: float
     ms_phases: float
+
+
+class _QueryStats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("graph", C.c_uint32), ("n", C.c_uint64), ("ms_device", C.c_double),
+        ("ms_latency", C.c_double), ("similar_edges", C.c_uint64), ("cores", C.c_uint64), ("clusters", C.c_uint64),
+        ("borders", C.c_uint64), ("hubs", C.c_uint64), ("outliers", C.c_uint64),
+    ]
+
+
+@dataclass
+class QueryStats:
+    status: int
+    graph: int
+    n: int
+    ms_device: float
+    ms_latency: float
+    similar_edges: int
+    cores: int
+    clusters: int
+    borders: int
+    hubs: int
+    outliers: int
 
 
 class _GraphT(C.Structure):
@@ -174,6 +197,16 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_dgraph_m", C.c_uint64, [_VP])
     _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_ooc_min_budget", C.c_int, [_VP, C.c_double, _P(C.c_uint64)])
+    _sig(lib, "sgc_server_create", C.c_int, [_VP, C.c_uint64, C.c_uint32, _P(_VP)])
+    _sig(lib, "sgc_server_destroy", None, [_VP])
+    _sig(lib, "sgc_server_graph_bytes", C.c_uint64, [_VP, C.c_uint32])
+    _sig(lib, "sgc_server_bytes_in_use", C.c_uint64, [_VP])
+    _sig(lib, "sgc_server_add_graph", C.c_int, [_VP, _VP, _P(C.c_uint32), _P(C.c_double)])
+    _sig(lib, "sgc_server_remove_graph", C.c_int, [_VP, C.c_uint32])
+    _sig(lib, "sgc_server_submit", C.c_int, [_VP, C.c_uint32, C.c_double, C.c_int32, _P(C.c_uint64)])
+    _sig(lib, "sgc_server_progress", C.c_int, [_VP, C.c_int, _P(C.c_uint32)])
+    _sig(lib, "sgc_server_pending", C.c_uint32, [_VP])
+    _sig(lib, "sgc_server_result", C.c_int, [_VP, C.c_uint64, _VP, C.c_uint64, _P(_QueryStats)])
     _sig(lib, "sgc_scan_ooc", C.c_int,
          [_VP, _VP, C.c_double, C.c_int32, C.c_uint64, _VP, _VP, _VP, _P(_Stats), _P(_OocStats)])
     _sig(lib, "sgc_scan_device", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
@@ -439,6 +472,78 @@ def copy_bench(ctx: Context, nbytes: int) -> float:
     ms = C.c_double()
     _check(lib.sgc_copy_bench(ctx.handle, int(nbytes), C.byref(ms)))
     return ms.value
+
+
+class Server:
+    """Serving engine: resident graphs with precomputed common-neighbour
+    counts, many concurrent (graph, eps, mu) queries."""
+
+    EBUSY = 7
+
+    def __init__(self, ctx: Context, hbm_budget: int, slots_per_graph: int = 4):
+        self._lib = load_library()
+        self._ctx = ctx
+        h = _VP()
+        _check(self._lib.sgc_server_create(ctx.handle, int(hbm_budget), int(slots_per_graph), C.byref(h)))
+        self._h = h.value
+        self.slots_per_graph = slots_per_graph
+        self._n = {}
+
+    @staticmethod
+    def graph_bytes(g: HostGraph, slots_per_graph: int = 4) -> int:
+        return int(load_library().sgc_server_graph_bytes(g._h, int(slots_per_graph)))
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(self._lib.sgc_server_bytes_in_use(self._h))
+
+    def add_graph(self, g: HostGraph) -> tuple[int, float]:
+        gid = C.c_uint32()
+        ms = C.c_double()
+        _check(self._lib.sgc_server_add_graph(self._h, g._h, C.byref(gid), C.byref(ms)))
+        self._n[gid.value] = g.n
+        return gid.value, ms.value
+
+    def remove_graph(self, gid: int) -> None:
+        _check(self._lib.sgc_server_remove_graph(self._h, int(gid)))
+
+    def submit(self, gid: int, eps: float, mu: int) -> int:
+        t = C.c_uint64()
+        _check(self._lib.sgc_server_submit(self._h, int(gid), float(eps), int(mu), C.byref(t)))
+        return t.value
+
+    def progress(self, wait: bool = False) -> int:
+        done = C.c_uint32()
+        _check(self._lib.sgc_server_progress(self._h, 1 if wait else 0, C.byref(done)))
+        return done.value
+
+    @property
+    def pending(self) -> int:
+        return int(self._lib.sgc_server_pending(self._h))
+
+    def result(self, ticket: int, n: int):
+        """(labels, QueryStats) of a retired query, or None while it runs."""
+        labels = np.empty(max(n, 1), dtype=np.int32)
+        st = _QueryStats()
+        rc = self._lib.sgc_server_result(self._h, int(ticket), _ptr(labels), labels.size, C.byref(st))
+        if rc == self.EBUSY:
+            return None
+        _check(rc)
+        return labels[:n], QueryStats(**{f.name: getattr(st, f.name) for f in fields(QueryStats)})
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sgc_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def GPUScan(folder: str, eps: float, miu: int) -> np.ndarray:

@@ -108,6 +108,9 @@ struct ScanParams {
   int32_t mu;
 };
 
+// Carves the arrays of one pass over g from the context's work buffer.
+int carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb);
+
 // Units per tier are bounded by the number of owned undecided edges, which is
 // at most m; this is the capacity each unit array is allocated with.
 inline size_t unit_capacity(uint64_t m) { return (size_t)m + 1; }
@@ -119,11 +122,15 @@ size_t scan_tmp_words(uint64_t nitems);
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
 void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+// count_all: every edge is left to its owner (for exact common-neighbour
+// counts) instead of the degree-only decision.
+void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all = false);
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-// `mid` (optional) is recorded between the workgroup-tier and wave-tier kernels.
+// `mid` (optional) is recorded between the workgroup-tier and wave-tier
+// kernels.  With cn_out the tiered kernels count every owned edge fully and
+// write |N(a) & N(b)| to both of its entries instead of flags.
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
-                       hipStream_t s);
+                       hipStream_t s, uint32_t* cn_out = nullptr);
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
 void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
 void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);

@@ -83,6 +83,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+int sgc::carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) { return carve(ctx, g, pb); }
+
 extern "C" int sgc_ctx_create(int device, sgc_ctx** out) {
   clear_error();
   if (!out) return fail(SGC_EINVAL, "null argument");

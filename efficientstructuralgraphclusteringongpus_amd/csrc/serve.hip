@@ -1,0 +1,434 @@
+// Serving engine: many SCAN queries (graph, eps, mu) against resident graphs.
+// No counterpart in the reference, which runs one query per process
+// (GPUSCAN++/main.cpp:20-111); the per-query semantics are the reference's
+// (GPUSCAN++/GPUScan.cu:133-798, canonical borders).
+//
+// A graph is admitted if it fits the engine's HBM budget: its CSR, the row of
+// each entry, cn[] and a fixed number of query slots.  At admission every
+// edge's common-neighbour count |N(u) & N(v)| is computed once by the tiered
+// similarity kernels in their counting mode.  A query is then a threshold
+// pass over cn[] (the degree-only rule, then the reference's predicate in
+// its int32/double form) followed by the core, union-find, border and
+// classify kernels, all O(m) streaming passes on the slot's stream.
+// Continuous batching: queries take free slots of their graph as they arrive
+// and are retired as their streams finish; queued queries start in the freed
+// slots, so several queries of different graphs overlap on the device.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <deque>
+#include <unordered_map>
+#include <vector>
+
+#include "device.h"
+#include "device_util.h"
+#include "graph_host.h"
+
+namespace sgc {
+namespace {
+
+constexpr int kStreams = 8;
+
+// flag[k] for one query from cn[]: the degree-only rule first
+// (GPUSCAN++/GPUScan.cu:143-169), then for undecided edges the predicate
+// `(cn+2)*(cn+2) < eps*eps*(s+1)*(l+1)` -> not similar (GPUScan.cu:220-221).
+__global__ void __launch_bounds__(kBlock) k_threshold(uint32_t m, const uint32_t* __restrict__ rp,
+                                                      const uint32_t* __restrict__ src,
+                                                      const int32_t* __restrict__ adj, const uint32_t* __restrict__ cn,
+                                                      double eps, uint8_t* __restrict__ flag) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
+    const uint32_t u = src[k], v = (uint32_t)adj[k];
+    const uint32_t du = rp[u + 1] - rp[u], dv = rp[v + 1] - rp[v];
+    uint8_t f = prep_flag(u, v, du, dv, eps);
+    if (f >= kFlagUUp) {
+      const uint32_t s = f == kFlagUUp ? du : dv, l = f == kFlagUUp ? dv : du;
+      f = predicate_exact(cn[k], predicate_rhs(eps, s, l)) ? kFlagY : kFlagN;
+    }
+    flag[k] = f;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_query_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
+                                                       int32_t* __restrict__ hi, uint64_t* __restrict__ cnt) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    sd[x] = 0;
+    lo[x] = kNoLabel;
+    hi[x] = -1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Device bytes of one query slot for a graph of n vertices, m entries.
+size_t slot_bytes(uint64_t n, uint64_t m) {
+  return align256(std::max<uint64_t>(m, 1)) + 5 * align256(n * 4 + 4) + align256(n + 1) + align256(kCntSlots * 8);
+}
+
+struct Slot {
+  bool busy = false;
+  uint64_t ticket = 0;
+  int stream = 0;
+  PassBuffers pb{};
+  int32_t* h_labels = nullptr;  // pinned
+  uint64_t* h_cnt = nullptr;    // pinned
+  hipEvent_t start = nullptr, done = nullptr;
+  double t_submit = 0;
+};
+
+struct Graph {
+  uint32_t id = 0;
+  sgc_dgraph* dg = nullptr;
+  uint32_t* cn = nullptr;
+  void* slot_mem = nullptr;
+  size_t bytes = 0;
+  std::vector<Slot> slots;
+  double precompute_ms = 0;
+};
+
+struct Query {
+  uint64_t ticket;
+  uint32_t graph;
+  double eps;
+  int32_t mu;
+  double t_submit;
+};
+
+struct Result {
+  std::vector<int32_t> labels;
+  sgc_query_stats st;
+};
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
+}
+
+}  // namespace
+}  // namespace sgc
+
+struct sgc_server {
+  sgc_ctx* ctx = nullptr;
+  uint64_t budget = 0, used = 0;
+  uint32_t slots_per_graph = 0;
+  uint32_t next_graph = 1;
+  uint64_t next_ticket = 1;
+  hipStream_t streams[sgc::kStreams] = {};
+  int next_stream = 0;
+  std::unordered_map<uint32_t, sgc::Graph*> graphs;
+  std::deque<sgc::Query> pending;
+  std::unordered_map<uint64_t, sgc::Result> done;
+  std::unordered_map<uint64_t, uint32_t> inflight;  // ticket -> graph
+};
+
+namespace sgc {
+namespace {
+
+void free_graph(Graph* G) {
+  for (Slot& s : G->slots) {
+    if (s.h_labels) (void)hipHostFree(s.h_labels);
+    if (s.h_cnt) (void)hipHostFree(s.h_cnt);
+    if (s.start) (void)hipEventDestroy(s.start);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  if (G->slot_mem) (void)hipFree(G->slot_mem);
+  if (G->cn) (void)hipFree(G->cn);
+  if (G->dg) sgc_dgraph_free(G->dg);
+  delete G;
+}
+
+// Launches query q into free slot s of graph G.
+int launch_query(sgc_server* S, Graph* G, Slot& sl, const Query& q) {
+  hipStream_t st = S->streams[sl.stream];
+  const sgc_dgraph* g = G->dg;
+  const ScanParams p{q.eps, q.mu};
+  SGC_HIP(hipEventRecord(sl.start, st));
+  k_query_init<<<grid_for(g->n, kBlock), kBlock, 0, st>>>(g->n, sl.pb.sd, sl.pb.lo, sl.pb.hi, sl.pb.cnt);
+  if (g->m)
+    k_threshold<<<grid_for(g->m, kBlock), kBlock, 0, st>>>(g->m, g->row_ptr, g->src, g->adj, G->cn, q.eps,
+                                                            sl.pb.flag);
+  launch_core(g, sl.pb, p, st);
+  launch_cluster(g, sl.pb, st);
+  launch_border(g, sl.pb, st);
+  launch_classify(g, sl.pb, p, st);
+  SGC_HIP(hipGetLastError());
+  SGC_HIP(hipMemcpyAsync(sl.h_labels, sl.pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, st));
+  SGC_HIP(hipMemcpyAsync(sl.h_cnt, sl.pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, st));
+  SGC_HIP(hipEventRecord(sl.done, st));
+  sl.busy = true;
+  sl.ticket = q.ticket;
+  sl.t_submit = q.t_submit;
+  S->inflight[q.ticket] = G->id;
+  return SGC_OK;
+}
+
+// Starts queued queries in free slots (FIFO per graph, skipping queries whose
+// graph is full so other graphs keep the device busy).
+int dispatch(sgc_server* S) {
+  for (auto it = S->pending.begin(); it != S->pending.end();) {
+    auto gi = S->graphs.find(it->graph);
+    if (gi == S->graphs.end()) {  // graph removed: drop with an error result
+      Result r;
+      memset(&r.st, 0, sizeof(r.st));
+      r.st.status = SGC_EINVAL;
+      S->done.emplace(it->ticket, std::move(r));
+      it = S->pending.erase(it);
+      continue;
+    }
+    Graph* G = gi->second;
+    Slot* free_slot = nullptr;
+    for (Slot& s : G->slots)
+      if (!s.busy) {
+        free_slot = &s;
+        break;
+      }
+    if (!free_slot) {
+      ++it;
+      continue;
+    }
+    int rc = launch_query(S, G, *free_slot, *it);
+    if (rc) return rc;
+    it = S->pending.erase(it);
+  }
+  return SGC_OK;
+}
+
+// Retires finished slots; with wait, blocks for the oldest in-flight query
+// first.  Returns the number retired.
+int retire(sgc_server* S, bool wait, int* retired) {
+  *retired = 0;
+  if (wait) {
+    // the in-flight query submitted first
+    Slot* oldest = nullptr;
+    for (auto& kv : S->graphs)
+      for (Slot& s : kv.second->slots)
+        if (s.busy && (!oldest || s.ticket < oldest->ticket)) oldest = &s;
+    if (oldest) SGC_HIP(hipEventSynchronize(oldest->done));
+  }
+  for (auto& kv : S->graphs) {
+    Graph* G = kv.second;
+    for (Slot& s : G->slots) {
+      if (!s.busy) continue;
+      const hipError_t e = hipEventQuery(s.done);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(SGC_EDEVICE, "query failed: %s", hipGetErrorString(e));
+      Result r;
+      r.labels.assign(s.h_labels, s.h_labels + G->dg->n);
+      memset(&r.st, 0, sizeof(r.st));
+      r.st.status = SGC_OK;
+      r.st.graph = G->id;
+      r.st.n = G->dg->n;
+      r.st.ms_device = elapsed(s.start, s.done);
+      r.st.ms_latency = now_ms() - s.t_submit;
+      const uint64_t* c = s.h_cnt;
+      r.st.similar_edges = c[kCntSimilar] / 2;
+      r.st.cores = c[kCntCores];
+      r.st.clusters = c[kCntClusters];
+      r.st.borders = c[kCntBorders];
+      r.st.hubs = c[kCntHubs];
+      r.st.outliers = c[kCntOutliers];
+      S->done.emplace(s.ticket, std::move(r));
+      S->inflight.erase(s.ticket);
+      s.busy = false;
+      ++*retired;
+    }
+  }
+  return SGC_OK;
+}
+
+}  // namespace
+}  // namespace sgc
+
+using namespace sgc;
+
+extern "C" int sgc_server_create(sgc_ctx* ctx, uint64_t hbm_budget, uint32_t slots_per_graph, sgc_server** out) {
+  clear_error();
+  if (!ctx || !out || slots_per_graph == 0 || slots_per_graph > 64) return fail(SGC_EINVAL, "bad argument");
+  SGC_HIP(hipSetDevice(ctx->device));
+  sgc_server* S = new sgc_server();
+  S->ctx = ctx;
+  S->budget = hbm_budget;
+  S->slots_per_graph = slots_per_graph;
+  for (int i = 0; i < kStreams; ++i) {
+    if (hipStreamCreateWithFlags(&S->streams[i], hipStreamNonBlocking) != hipSuccess) {
+      for (int j = 0; j < i; ++j) (void)hipStreamDestroy(S->streams[j]);
+      delete S;
+      return fail(SGC_EDEVICE, "stream creation failed");
+    }
+  }
+  *out = S;
+  return SGC_OK;
+}
+
+extern "C" void sgc_server_destroy(sgc_server* S) {
+  if (!S) return;
+  (void)hipSetDevice(S->ctx->device);
+  for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(S->streams[i]);
+  for (auto& kv : S->graphs) free_graph(kv.second);
+  for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(S->streams[i]);
+  delete S;
+}
+
+extern "C" uint64_t sgc_server_graph_bytes(const sgc_host_graph* g, uint32_t slots_per_graph) {
+  if (!g) return 0;
+  const uint64_t n = g->n, m = g->m;
+  // row_ptr, adj, src, cn, similarity scratch of the admission pass, slots
+  const uint64_t graph = align256((n + 1) * 4) + 3 * align256(std::max<uint64_t>(m, 1) * 4);
+  return graph + slots_per_graph * slot_bytes(n, m);
+}
+
+extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uint32_t* graph_id,
+                                    double* precompute_ms) {
+  clear_error();
+  if (!S || !hg || !graph_id) return fail(SGC_EINVAL, "null argument");
+  const uint64_t need = sgc_server_graph_bytes(hg, S->slots_per_graph);
+  if (S->used + need > S->budget)
+    return fail(SGC_ENOMEM, "admission refused: graph needs %llu bytes, %llu of %llu in use",
+                (unsigned long long)need, (unsigned long long)S->used, (unsigned long long)S->budget);
+  sgc_ctx* ctx = S->ctx;
+  SGC_HIP(hipSetDevice(ctx->device));
+  Graph* G = new Graph();
+  int rc = sgc_dgraph_upload(ctx, hg, &G->dg);
+  if (rc) {
+    delete G;
+    return rc;
+  }
+  const uint32_t n = G->dg->n, m = G->dg->m;
+  if (hipMalloc(&G->cn, std::max<size_t>(m, 1) * 4) != hipSuccess ||
+      hipMalloc(&G->slot_mem, S->slots_per_graph * slot_bytes(n, m)) != hipSuccess) {
+    (void)hipGetLastError();
+    free_graph(G);
+    return fail(SGC_ENOMEM, "device allocation for graph of %u vertices failed", n);
+  }
+  // exact common-neighbour counts of every edge, once
+  const double t0 = now_ms();
+  if (m) {
+    PassBuffers pb;
+    rc = carve_pass_buffers(ctx, G->dg, &pb);
+    if (rc) {
+      free_graph(G);
+      return rc;
+    }
+    const ScanParams p{1.0, 2};
+    hipStream_t s = ctx->stream;
+    launch_init(G->dg, pb, s);
+    launch_prep(G->dg, pb, p, s, true);
+    launch_plan(G->dg, pb, s);
+    launch_similarity(G->dg, pb, p, ctx->cu_count, kSimTiered, nullptr, s, G->cn);
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess || hipGetLastError() != hipSuccess) {
+      free_graph(G);
+      return fail(SGC_EDEVICE, "common-neighbour pass failed: %s", hipGetErrorString(e));
+    }
+  }
+  G->precompute_ms = now_ms() - t0;
+  // query slots
+  G->slots.resize(S->slots_per_graph);
+  char* base = (char*)G->slot_mem;
+  for (uint32_t i = 0; i < S->slots_per_graph; ++i) {
+    Slot& sl = G->slots[i];
+    char* p = base + i * slot_bytes(n, m);
+    sl.pb.flag = (uint8_t*)p;
+    p += align256(std::max<uint32_t>(m, 1));
+    sl.pb.sd = (uint32_t*)p;
+    p += align256((size_t)n * 4 + 4);
+    sl.pb.parent = (int32_t*)p;
+    p += align256((size_t)n * 4 + 4);
+    sl.pb.label = (int32_t*)p;
+    p += align256((size_t)n * 4 + 4);
+    sl.pb.lo = (int32_t*)p;
+    p += align256((size_t)n * 4 + 4);
+    sl.pb.hi = (int32_t*)p;
+    p += align256((size_t)n * 4 + 4);
+    sl.pb.core = (uint8_t*)p;
+    p += align256((size_t)n + 1);
+    sl.pb.cnt = (uint64_t*)p;
+    sl.stream = S->next_stream;
+    S->next_stream = (S->next_stream + 1) % kStreams;
+    if (hipHostMalloc((void**)&sl.h_labels, std::max<size_t>(n, 1) * 4) != hipSuccess ||
+        hipHostMalloc((void**)&sl.h_cnt, kCntSlots * 8) != hipSuccess ||
+        hipEventCreate(&sl.start) != hipSuccess || hipEventCreate(&sl.done) != hipSuccess) {
+      (void)hipGetLastError();
+      free_graph(G);
+      return fail(SGC_ENOMEM, "query slot set-up failed");
+    }
+  }
+  G->id = S->next_graph++;
+  G->bytes = need;
+  S->used += need;
+  S->graphs[G->id] = G;
+  *graph_id = G->id;
+  if (precompute_ms) *precompute_ms = G->precompute_ms;
+  return SGC_OK;
+}
+
+extern "C" int sgc_server_remove_graph(sgc_server* S, uint32_t graph_id) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  auto it = S->graphs.find(graph_id);
+  if (it == S->graphs.end()) return fail(SGC_EINVAL, "unknown graph %u", graph_id);
+  Graph* G = it->second;
+  for (Slot& s : G->slots)
+    if (s.busy) SGC_HIP(hipEventSynchronize(s.done));
+  int retired = 0;
+  int rc = retire(S, false, &retired);  // keep the finished results
+  if (rc) return rc;
+  S->used -= G->bytes;
+  S->graphs.erase(it);
+  free_graph(G);
+  return SGC_OK;
+}
+
+extern "C" uint64_t sgc_server_bytes_in_use(const sgc_server* S) { return S ? S->used : 0; }
+
+extern "C" int sgc_server_submit(sgc_server* S, uint32_t graph_id, double eps, int32_t mu, uint64_t* ticket) {
+  clear_error();
+  if (!S || !ticket) return fail(SGC_EINVAL, "null argument");
+  if (!S->graphs.count(graph_id)) return fail(SGC_EINVAL, "unknown graph %u", graph_id);
+  SGC_HIP(hipSetDevice(S->ctx->device));
+  Query q{S->next_ticket++, graph_id, eps, mu, now_ms()};
+  S->pending.push_back(q);
+  *ticket = q.ticket;
+  return dispatch(S);
+}
+
+extern "C" int sgc_server_progress(sgc_server* S, int wait, uint32_t* completed) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  SGC_HIP(hipSetDevice(S->ctx->device));
+  int retired = 0;
+  int rc = retire(S, wait && !S->inflight.empty(), &retired);
+  if (rc) return rc;
+  rc = dispatch(S);
+  if (completed) *completed = (uint32_t)retired;
+  return rc;
+}
+
+extern "C" uint32_t sgc_server_pending(const sgc_server* S) {
+  return S ? (uint32_t)(S->pending.size() + S->inflight.size()) : 0;
+}
+
+extern "C" int sgc_server_result(sgc_server* S, uint64_t ticket, int32_t* labels, uint64_t capacity,
+                                 sgc_query_stats* st) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  auto it = S->done.find(ticket);
+  if (it == S->done.end()) {
+    if (S->inflight.count(ticket)) return SGC_EBUSY;
+    for (const Query& q : S->pending)
+      if (q.ticket == ticket) return SGC_EBUSY;
+    return fail(SGC_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+  }
+  const Result& r = it->second;
+  if (labels) {
+    if (capacity < r.labels.size()) return fail(SGC_EINVAL, "labels buffer holds %llu < %zu", (unsigned long long)capacity, r.labels.size());
+    memcpy(labels, r.labels.data(), r.labels.size() * 4);
+  }
+  if (st) *st = r.st;
+  const int status = r.st.status;
+  S->done.erase(it);
+  return status == SGC_OK ? SGC_OK : fail(status, "query on a removed graph");
+}

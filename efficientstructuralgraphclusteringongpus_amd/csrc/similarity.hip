@@ -76,7 +76,7 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                  const uint32_t* __restrict__ src,
-                                                 double eps, uint8_t* __restrict__ flag,
+                                                 double eps, bool count_all, uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
                                                  uint64_t* __restrict__ cnt) {
   const uint32_t lane = lane_id();
@@ -92,7 +92,11 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       row = u;
       const uint32_t v = (uint32_t)adj[k];
       const uint32_t dv = re[v] - rb[v];
-      const uint8_t f = prep_flag(u, v, re[u] - rb[u], dv, eps);
+      uint8_t f = prep_flag(u, v, re[u] - rb[u], dv, eps);
+      if (count_all && f <= kFlagY) {  // every edge to its owner, for exact counts
+        const uint32_t du = re[u] - rb[u];
+        f = (dv > du || (dv == du && v > u)) ? kFlagUUp : kFlagUDown;
+      }
       flag[k] = f;
       if (f == kFlagUDown) {
         owned = true;
@@ -463,6 +467,7 @@ __global__ void __launch_bounds__(T::kThreads)
     k_sim_units(uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
                 const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
+                uint32_t* __restrict__ cn_out,
                 const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
                 int units_slot, int queue_slot) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
@@ -497,7 +502,7 @@ __global__ void __launch_bounds__(T::kThreads)
         ab = rb[a];
         s = re[a] - ab;
         const double rhs = predicate_rhs(eps, s, l);
-        cstar = s <= kMaxThresholdRow ? min_common_threshold(rhs, s) : kNone;
+        cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
       }
       e_k[t] = k;
       e_ab[t] = ab;
@@ -636,9 +641,14 @@ __global__ void __launch_bounds__(T::kThreads)
           }
           mpos = lo < ab + s && adj[lo] == (int32_t)b ? lo : kNone;
         }
-        const uint8_t f = sim ? kFlagY : kFlagN;
-        flag[k] = f;
-        if (mpos != kNone) flag[mpos] = f;
+        if (cn_out) {  // exact counts instead of flags
+          cn_out[k] = hits;
+          if (mpos != kNone) cn_out[mpos] = hits;
+        } else {
+          const uint8_t f = sim ? kFlagY : kFlagN;
+          flag[k] = f;
+          if (mpos != kNone) flag[mpos] = f;
+        }
       }
     }
     __syncthreads();
@@ -652,10 +662,10 @@ __global__ void __launch_bounds__(T::kThreads)
 
 }  // namespace
 
-void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag,
-                                                   b.cnt_u, b.work_u, b.cnt);
+  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, count_all,
+                                                   b.flag, b.cnt_u, b.work_u, b.cnt);
 }
 
 static void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t nitems, uint32_t* tmp, hipStream_t s) {
@@ -677,21 +687,21 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
-                       hipStream_t s) {
+                       hipStream_t s, uint32_t* cn_out) {
   if (g->m == 0) return;
-  if (impl == kSimWavePerEdge) {
+  if (impl == kSimWavePerEdge && !cn_out) {
     launch_similarity_wave(g, b, p, cu_count, s);
     if (mid) (void)hipEventRecord(mid, s);
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
-                                                             b.flag,
-                                                             b.ulist, b.units2, b.cnt, kCntUnits2, kCntQueue2);
+                                                             b.flag, cn_out, b.ulist, b.units2, b.cnt, kCntUnits2,
+                                                             kCntQueue2);
   if (mid) (void)hipEventRecord(mid, s);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src,
-                                                                 p.eps, b.flag,
-                                                                 b.ulist, b.units1, b.cnt, kCntUnits1, kCntQueue1);
+                                                                 p.eps, b.flag, cn_out, b.ulist, b.units1, b.cnt,
+                                                                 kCntUnits1, kCntQueue1);
 }
 
 }  // namespace sgc

@@ -27,6 +27,7 @@ extern "C" {
 #define SGC_ENOMEM 4  /* host/device allocation failed or budget too small */
 #define SGC_EDEVICE 5 /* HIP runtime error */
 #define SGC_ERANGE 6  /* size beyond what this entry point supports */
+#define SGC_EBUSY 7   /* a served query has not finished yet */
 
 #define SGC_HUB (-1)
 #define SGC_OUTLIER (-2)
@@ -234,6 +235,52 @@ int sgc_scan_ooc(sgc_ctx* ctx, const sgc_host_graph* g, double eps, int32_t mu, 
  * short rows of its undecided edges).  No counterpart in the reference, whose
  * chip planner takes a fixed budget and fails inside. */
 int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t* bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Serving engine (no counterpart in the reference, which answers one query  */
+/* per process, GPUSCAN++/main.cpp:20-111)                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sgc_server sgc_server;
+
+typedef struct {
+  int32_t status;        /* SGC_OK, or the error of a query whose graph was removed */
+  uint32_t graph;
+  uint64_t n;
+  double ms_device;      /* device time of the query on its stream */
+  double ms_latency;     /* submit -> retired by sgc_server_progress */
+  uint64_t similar_edges;
+  uint64_t cores;
+  uint64_t clusters;
+  uint64_t borders;
+  uint64_t hubs;
+  uint64_t outliers;
+} sgc_query_stats;
+
+/* An engine holding resident graphs within `hbm_budget` bytes of device
+ * memory, each with `slots_per_graph` concurrent query slots (1..64). */
+int sgc_server_create(sgc_ctx* ctx, uint64_t hbm_budget, uint32_t slots_per_graph, sgc_server** out);
+void sgc_server_destroy(sgc_server* srv);
+/* Device bytes a graph takes in an engine (CSR, cn[], query slots). */
+uint64_t sgc_server_graph_bytes(const sgc_host_graph* g, uint32_t slots_per_graph);
+uint64_t sgc_server_bytes_in_use(const sgc_server* srv);
+/* Admission: uploads g and computes every edge's common-neighbour count once
+ * (precompute_ms: that pass).  SGC_ENOMEM if the budget cannot hold it; the
+ * caller may remove graphs and retry. */
+int sgc_server_add_graph(sgc_server* srv, const sgc_host_graph* g, uint32_t* graph_id, double* precompute_ms);
+/* Waits for the graph's running queries; queued ones end with SGC_EINVAL. */
+int sgc_server_remove_graph(sgc_server* srv, uint32_t graph_id);
+/* Queues a query (eps, mu as sgc_scan) and starts it at once if one of the
+ * graph's slots is free. */
+int sgc_server_submit(sgc_server* srv, uint32_t graph_id, double eps, int32_t mu, uint64_t* ticket);
+/* Retires finished queries and starts queued ones in the freed slots; with
+ * wait, first blocks until the oldest running query finishes. */
+int sgc_server_progress(sgc_server* srv, int wait, uint32_t* completed);
+/* Queries submitted and not yet retired. */
+uint32_t sgc_server_pending(const sgc_server* srv);
+/* A retired query's labels (n int32, as sgc_scan) and statistics; the result
+ * is then released.  SGC_EBUSY while it is queued or running. */
+int sgc_server_result(sgc_server* srv, uint64_t ticket, int32_t* labels, uint64_t capacity, sgc_query_stats* stats);
 
 /* Reference-layout graph (GPUSCAN++/Graph.h:13-20). */
 typedef struct {
