@@ -80,6 +80,107 @@ def cpu_baseline(g, eps_list, seconds: float):
     }
 
 
+def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
+    """Serving workload: 32 resident R-MAT graphs, n log-uniform in [10k, 5M]
+    (rounded to a power of two: scales 13..22), edge factor uniform in
+    [8, 32], seeds 1..32; queries (graph uniform, eps uniform on
+    {0.1, ..., 0.9}, mu uniform on {2..20}) from a fixed-seed stream.
+    Peak qps from a closed-loop run of `queries` queries; p50/p99 latency
+    (arrival -> labels on the host) under Poisson arrivals at half the peak;
+    the one-shot rate runs full plain SCAN passes (labels to the host) for a
+    sample of the same queries on the same resident graphs."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    graphs = []
+    for i in range(1, 33):
+        n = float(np.exp(rng.uniform(np.log(1e4), np.log(5e6))))
+        scale = int(min(22, max(13, round(np.log2(n)))))
+        ef = int(rng.integers(8, 33))
+        graphs.append(sgc.HostGraph.rmat(scale, ef, seed=i))
+    gen_s = time.perf_counter() - t0
+    slots = 4
+    need = sum(sgc.Server.graph_bytes(g, slots) for g in graphs)
+    srv = sgc.Server(ctx, need + (1 << 30), slots_per_graph=slots)
+    t0 = time.perf_counter()
+    ids, pre_ms = [], 0.0
+    for g in graphs:
+        gid, ms = srv.add_graph(g)
+        ids.append(gid)
+        pre_ms += ms
+    admit_s = time.perf_counter() - t0
+    E = sum(g.m for g in graphs) // 2
+    log(f"serving: 32 graphs, {E} edges, generated in {gen_s:.1f} s, admitted (upload + common-neighbour "
+        f"counts) in {admit_s:.1f} s (counting {pre_ms / 1e3:.2f} s)")
+    qrng = np.random.default_rng(seed + 1)
+    stream = [(int(qrng.integers(0, 32)), float(qrng.integers(1, 10)) / 10.0, int(qrng.integers(2, 21)))
+              for _ in range(queries)]
+
+    def run(qs, arrivals=None):
+        lat = []
+        open_ = {}
+        nxt = 0
+        t_start = time.perf_counter()
+        while nxt < len(qs) or open_:
+            now = time.perf_counter() - t_start
+            while nxt < len(qs) and (arrivals is None or arrivals[nxt] <= now):
+                gi, eps, mu = qs[nxt]
+                arr = t_start + (arrivals[nxt] if arrivals is not None else 0.0)
+                open_[srv.submit(ids[gi], eps, mu)] = (graphs[gi].n, arr)
+                nxt += 1
+            idle = arrivals is not None and nxt < len(qs) and not open_
+            if srv.progress(wait=bool(open_) and (arrivals is None or nxt >= len(qs))):
+                for t in list(open_):
+                    r = srv.result(t, open_[t][0])
+                    if r is not None:
+                        lat.append((time.perf_counter() - open_.pop(t)[1]) * 1e3)
+            if idle:
+                time.sleep(0.0002)
+        return time.perf_counter() - t_start, lat
+
+    run(stream[: min(64, len(stream))])  # warm-up
+    wall, _ = run(stream)
+    qps = len(stream) / wall
+    offered = qps / 2
+    arrivals = np.cumsum(np.random.default_rng(seed + 2).exponential(1.0 / offered, size=len(stream)))
+    wall2, lat = run(stream, arrivals)
+    lat = np.array(lat)
+    sample = stream[: min(100, len(stream))]
+    dgs = {}
+    t0 = time.perf_counter()
+    for gi, eps, mu in sample:
+        if gi not in dgs:
+            dgs[gi] = sgc.DeviceGraph(ctx, graphs[gi])
+    t0 = time.perf_counter()
+    for gi, eps, mu in sample:
+        sgc.scan(ctx, dgs[gi], eps, mu)
+    one_shot = len(sample) / (time.perf_counter() - t0)
+    for d in dgs.values():
+        d.close()
+    srv.close()
+    out = {
+        "qps": round(qps, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "offered_qps": round(offered, 1),
+        "achieved_qps_at_offered": round(len(stream) / wall2, 1),
+        "one_shot_qps": round(one_shot, 1),
+        "speedup_vs_one_shot": round(qps / one_shot, 1),
+        "queries": len(stream),
+        "graphs": 32,
+        "edges": E,
+        "slots_per_graph": slots,
+        "admission_s": round(admit_s, 2),
+        "workload": ("32 R-MAT graphs (scales 13..22, ef 8..32, seeds 1..32), eps uniform on {0.1..0.9}, "
+                     "mu uniform on {2..20}; latency = arrival -> labels on the host, Poisson arrivals at half "
+                     "the closed-loop peak"),
+    }
+    log(f"serving: {qps:.0f} qps peak, p50 {out['p50_ms']} ms / p99 {out['p99_ms']} ms at {offered:.0f} qps "
+        f"offered; one-shot {one_shot:.1f} qps ({out['speedup_vs_one_shot']}x)")
+    return out
+
+
 def measured_traffic(args):
     """HBM bytes per launch of the workgroup-tier similarity kernel on the
     default workload, from the committed rocprofv3 counter summary
@@ -105,6 +206,8 @@ def main() -> int:
     ap.add_argument("--eps", default="0.2,0.4,0.6,0.8")
     ap.add_argument("--mu", default="2,5,10")
     ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget; 0 skips it")
+    ap.add_argument("--serve-queries", type=int, default=2000,
+                    help="queries of the serving run (32 resident graphs); 0 skips it")
     ap.add_argument("--copy-gib", type=float, default=1.0,
                     help="size of the plain device copy run first as the HBM counter yardstick; 0 skips it")
     args = ap.parse_args()
@@ -175,6 +278,12 @@ def main() -> int:
     phases = {k: sum(getattr(st, "ms_" + k) for _, _, st in stats) / len(stats)
               for k in ("prep", "similarity", "similarity_large", "core", "cluster", "border", "classify", "total")}
 
+    serving = None
+    if args.serve_queries > 0:
+        dg.close()
+        serving = serving_bench(sgc, ctx, args.serve_queries)
+        dg = None
+
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(g, eps_list, args.cpu_seconds)
@@ -224,11 +333,13 @@ def main() -> int:
             "streamed_bytes_per_launch": sum(4 * st.probe_elements for _, _, st in stats) / len(stats),
         },
         "cpu_baseline": cpu,
+        "serving": serving,
         "copy_kernel": copy,
         "phases_ms_avg": {k: round(v, 3) for k, v in phases.items()},
     }
     print(json.dumps(out), flush=True)
-    dg.close()
+    if dg is not None:
+        dg.close()
     ctx.close()
     return 0
 
