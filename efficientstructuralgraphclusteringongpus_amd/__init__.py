@@ -23,7 +23,8 @@ import numpy as np
 
 __all__ = [
     "SGCError", "HostGraph", "Context", "DeviceGraph", "ScanStats", "ScanResult",
-    "scan", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
+    "scan", "scan_device", "scan_ooc", "ooc_min_budget", "OocStats", "Server", "QueryStats", "Tuning",
+    "copy_bench", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
 ]
 
 HUB = -1
@@ -100,6 +101,21 @@ class QueryStats:
     borders: int
     hubs: int
     outliers: int
+
+
+class _Tuning(C.Structure):
+    _fields_ = [("small_max_degree", C.c_uint32), ("small_unit_work", C.c_double), ("large_unit_work", C.c_double),
+                ("ms_similarity", C.c_double), ("ms_default", C.c_double), ("trials", C.c_uint32)]
+
+
+@dataclass
+class Tuning:
+    small_max_degree: int = 1024
+    small_unit_work: float = 65536.0
+    large_unit_work: float = 1048576.0
+    ms_similarity: float = 0.0
+    ms_default: float = 0.0
+    trials: int = 0
 
 
 class _GraphT(C.Structure):
@@ -197,10 +213,14 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_dgraph_m", C.c_uint64, [_VP])
     _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_ooc_min_budget", C.c_int, [_VP, C.c_double, _P(C.c_uint64)])
+    _sig(lib, "sgc_dgraph_get_tuning", C.c_int, [_VP, _P(_Tuning)])
+    _sig(lib, "sgc_dgraph_set_tuning", C.c_int, [_VP, _P(_Tuning)])
+    _sig(lib, "sgc_dgraph_autotune", C.c_int, [_VP, _VP, C.c_double, _P(_Tuning)])
     _sig(lib, "sgc_server_create", C.c_int, [_VP, C.c_uint64, C.c_uint32, _P(_VP)])
     _sig(lib, "sgc_server_destroy", None, [_VP])
     _sig(lib, "sgc_server_graph_bytes", C.c_uint64, [_VP, C.c_uint32])
     _sig(lib, "sgc_server_bytes_in_use", C.c_uint64, [_VP])
+    _sig(lib, "sgc_server_set_graph_launch", C.c_int, [_VP, C.c_int])
     _sig(lib, "sgc_server_add_graph", C.c_int, [_VP, _VP, _P(C.c_uint32), _P(C.c_double)])
     _sig(lib, "sgc_server_remove_graph", C.c_int, [_VP, C.c_uint32])
     _sig(lib, "sgc_server_submit", C.c_int, [_VP, C.c_uint32, C.c_double, C.c_int32, _P(C.c_uint64)])
@@ -417,6 +437,24 @@ class DeviceGraph:
     def handle(self):
         return self._h
 
+    @property
+    def tuning(self) -> Tuning:
+        t = _Tuning()
+        _check(self._lib.sgc_dgraph_get_tuning(self._h, C.byref(t)))
+        return Tuning(**{f.name: getattr(t, f.name) for f in fields(Tuning)})
+
+    @tuning.setter
+    def tuning(self, value: Tuning) -> None:
+        t = _Tuning(value.small_max_degree, value.small_unit_work, value.large_unit_work, 0.0, 0.0, 0)
+        _check(self._lib.sgc_dgraph_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, eps: float) -> Tuning:
+        """Times the similarity phase at eps under candidate planner knobs and
+        keeps the fastest on this graph."""
+        t = _Tuning()
+        _check(self._lib.sgc_dgraph_autotune(self.ctx.handle, self._h, float(eps), C.byref(t)))
+        return Tuning(**{f.name: getattr(t, f.name) for f in fields(Tuning)})
+
 
 def scan(ctx: Context, dg: DeviceGraph, eps: float, mu: int, want_sim: bool = False) -> ScanResult:
     """One plain SCAN pass on a resident graph; returns host copies."""
@@ -496,6 +534,10 @@ class Server:
     @property
     def bytes_in_use(self) -> int:
         return int(self._lib.sgc_server_bytes_in_use(self._h))
+
+    def set_graph_launch(self, enable: bool) -> None:
+        """Replay a captured hipGraph per query (default) or launch eagerly."""
+        _check(self._lib.sgc_server_set_graph_launch(self._h, 1 if enable else 0))
 
     def add_graph(self, g: HostGraph) -> tuple[int, float]:
         gid = C.c_uint32()

@@ -64,6 +64,15 @@ struct sgc_ctx {
   int sim_impl = sgc::kSimTiered;
 };
 
+namespace sgc {
+// Planner knobs of the tiered similarity kernels (see sgc_tuning).
+struct SimTuning {
+  uint32_t small_max_degree = 1024;  // owners up to this degree take the small tier (<= 1024)
+  double small_unit_work = 65536.0;  // short-row elements per small-tier unit
+  double large_unit_work = 1048576.0;  // per workgroup-tier unit
+};
+}  // namespace sgc
+
 struct sgc_dgraph {
   sgc_ctx* ctx = nullptr;
   uint32_t n = 0;
@@ -79,6 +88,7 @@ struct sgc_dgraph {
   // read).
   const uint32_t* row_begin = nullptr;
   const uint32_t* row_end = nullptr;
+  sgc::SimTuning tuning;
 };
 
 namespace sgc {
@@ -140,11 +150,14 @@ void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hi
 // entries [0, g->m), vertex passes over all g->n vertices); the out-of-core
 // path runs the edge passes once per chip.
 void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+// mu_dev (optional): read mu from device memory instead of p (captured launches)
+void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
+                      const int32_t* mu_dev = nullptr);
 void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
+                     const int32_t* mu_dev = nullptr);
 void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 
 }  // namespace sgc

@@ -204,6 +204,28 @@ extern "C" void sgc_dgraph_free(sgc_dgraph* d) {
   delete d;
 }
 
+extern "C" int sgc_dgraph_get_tuning(const sgc_dgraph* d, sgc_tuning* t) {
+  clear_error();
+  if (!d || !t) return fail(SGC_EINVAL, "null argument");
+  memset(t, 0, sizeof(*t));
+  t->small_max_degree = d->tuning.small_max_degree;
+  t->small_unit_work = d->tuning.small_unit_work;
+  t->large_unit_work = d->tuning.large_unit_work;
+  return SGC_OK;
+}
+
+extern "C" int sgc_dgraph_set_tuning(sgc_dgraph* d, const sgc_tuning* t) {
+  clear_error();
+  if (!d || !t) return fail(SGC_EINVAL, "null argument");
+  if (t->small_max_degree < 1 || t->small_max_degree > 1024 || !(t->small_unit_work >= 64.0) ||
+      !(t->large_unit_work >= 64.0))
+    return fail(SGC_EINVAL, "tuning out of range (small_max_degree 1..1024, unit work >= 64)");
+  d->tuning.small_max_degree = t->small_max_degree;
+  d->tuning.small_unit_work = t->small_unit_work;
+  d->tuning.large_unit_work = t->large_unit_work;
+  return SGC_OK;
+}
+
 extern "C" uint64_t sgc_dgraph_n(const sgc_dgraph* d) { return d ? d->n : 0; }
 extern "C" uint64_t sgc_dgraph_m(const sgc_dgraph* d) { return d ? d->m : 0; }
 
@@ -260,6 +282,90 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
 }
 
 }  // namespace
+
+namespace {
+
+// Similarity-phase time of one pass (prep, plan and both tiers are run; the
+// similarity kernels alone are timed).
+int time_similarity(sgc_ctx* ctx, sgc_dgraph* g, double eps, PassBuffers* pb, double* ms) {
+  int rc = carve(ctx, g, pb);
+  if (rc) return rc;
+  const ScanParams p{eps, 2};
+  hipStream_t s = ctx->stream;
+  launch_init(g, *pb, s);
+  launch_prep(g, *pb, p, s);
+  launch_plan(g, *pb, s);
+  SGC_HIP(hipEventRecord(ctx->ev[1], s));
+  launch_similarity(g, *pb, p, ctx->cu_count, kSimTiered, nullptr, s);
+  SGC_HIP(hipEventRecord(ctx->ev[2], s));
+  SGC_HIP(hipGetLastError());
+  SGC_HIP(hipStreamSynchronize(s));
+  *ms = elapsed(ctx->ev[1], ctx->ev[2]);
+  return SGC_OK;
+}
+
+}  // namespace
+
+extern "C" int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* g, double eps, sgc_tuning* best) {
+  clear_error();
+  if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
+  SGC_HIP(hipSetDevice(ctx->device));
+  PassBuffers pb;
+  uint32_t trials = 0;
+  auto measure = [&](const SimTuning& t, double* ms) -> int {
+    g->tuning = t;
+    double a = 0, b = 0;
+    int rc = time_similarity(ctx, g, eps, &pb, &a);
+    if (!rc) rc = time_similarity(ctx, g, eps, &pb, &b);
+    trials += 2;
+    *ms = std::min(a, b);
+    return rc;
+  };
+  const SimTuning defaults;
+  double ms_default = 0;
+  int rc = measure(defaults, &ms_default);
+  if (rc) return rc;
+  SimTuning cur = defaults;
+  double cur_ms = ms_default;
+  const double large_work[] = {262144.0, 1048576.0, 4194304.0};
+  const uint32_t small_deg[] = {128, 256, 512, 1024};
+  const double small_work[] = {16384.0, 65536.0, 262144.0};
+  for (double v : large_work) {
+    SimTuning t = cur;
+    t.large_unit_work = v;
+    double ms;
+    if (v == cur.large_unit_work) continue;
+    if ((rc = measure(t, &ms))) return rc;
+    if (ms < cur_ms) cur = t, cur_ms = ms;
+  }
+  for (uint32_t v : small_deg) {
+    SimTuning t = cur;
+    t.small_max_degree = v;
+    double ms;
+    if (v == cur.small_max_degree) continue;
+    if ((rc = measure(t, &ms))) return rc;
+    if (ms < cur_ms) cur = t, cur_ms = ms;
+  }
+  for (double v : small_work) {
+    SimTuning t = cur;
+    t.small_unit_work = v;
+    double ms;
+    if (v == cur.small_unit_work) continue;
+    if ((rc = measure(t, &ms))) return rc;
+    if (ms < cur_ms) cur = t, cur_ms = ms;
+  }
+  g->tuning = cur;
+  if (best) {
+    memset(best, 0, sizeof(*best));
+    best->small_max_degree = cur.small_max_degree;
+    best->small_unit_work = cur.small_unit_work;
+    best->large_unit_work = cur.large_unit_work;
+    best->ms_similarity = cur_ms;
+    best->ms_default = ms_default;
+    best->trials = trials;
+  }
+  return SGC_OK;
+}
 
 extern "C" int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int32_t* d_labels,
                                uint8_t* d_core, uint8_t* d_sim, sgc_stats* st) {

@@ -150,9 +150,12 @@ __global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_
 // sd/ed state machine reaches exactly this fixed point: sd only counts
 // similar edges, ed only discounts dissimilar ones, and every edge of a
 // still-undecided vertex is evaluated.
-__global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t* __restrict__ sd, uint32_t thr,
-                                                      uint8_t* __restrict__ core, int32_t* __restrict__ parent,
-                                                      int32_t* __restrict__ label) {
+// mu comes from mu_dev when given (parameters of a captured launch), else mu.
+__global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t* __restrict__ sd, int32_t mu,
+                                                      const int32_t* __restrict__ mu_dev, uint8_t* __restrict__ core,
+                                                      int32_t* __restrict__ parent, int32_t* __restrict__ label) {
+  if (mu_dev) mu = *mu_dev;
+  const uint32_t thr = mu >= 1 ? (uint32_t)mu : 1u;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     const bool c = sd[x] >= thr;
@@ -268,9 +271,11 @@ __global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, const uint32_t*
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, const int32_t* __restrict__ lo,
+__global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, const int32_t* __restrict__ mu_dev,
+                                                     const int32_t* __restrict__ lo,
                                                      const int32_t* __restrict__ hi, const uint8_t* __restrict__ core,
                                                      int32_t* __restrict__ label, uint64_t* __restrict__ cnt) {
+  if (mu_dev) mu = *mu_dev;
   uint32_t cores = 0, clusters = 0, borders = 0, hubs = 0, outliers = 0;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
@@ -342,9 +347,9 @@ void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s)
   if (g->m) k_core_count<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.sd);
 }
 
-void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
-  const uint32_t thr = p.mu >= 1 ? (uint32_t)p.mu : 1u;
-  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, thr, b.core, b.parent, b.label);
+void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
+                      const int32_t* mu_dev) {
+  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, p.mu, mu_dev, b.core, b.parent, b.label);
 }
 
 void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
@@ -363,8 +368,9 @@ void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_
   if (g->m) k_classify<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.label, b.lo, b.hi);
 }
 
-void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
-  k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, b.lo, b.hi, b.core, b.label, b.cnt);
+void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
+                     const int32_t* mu_dev) {
+  k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, mu_dev, b.lo, b.hi, b.core, b.label, b.cnt);
 }
 
 void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {

@@ -13,6 +13,10 @@
 // Continuous batching: queries take free slots of their graph as they arrive
 // and are retired as their streams finish; queued queries start in the freed
 // slots, so several queries of different graphs overlap on the device.
+// Each slot's work is captured once into a hipGraph whose kernels read eps
+// and mu from the slot's device parameter block: a query is one 16-byte
+// parameter copy and one graph launch (the graph holds no memset nodes; the
+// per-query clearing is a kernel).  Eager launches remain selectable.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -30,13 +34,23 @@ namespace {
 
 constexpr int kStreams = 8;
 
+// Query parameters in device memory (captured launches read them).
+struct QueryParams {
+  double eps;
+  int32_t mu;
+  int32_t pad;
+};
+
 // flag[k] for one query from cn[]: the degree-only rule first
 // (GPUSCAN++/GPUScan.cu:143-169), then for undecided edges the predicate
 // `(cn+2)*(cn+2) < eps*eps*(s+1)*(l+1)` -> not similar (GPUScan.cu:220-221).
+// eps comes from prm when given.
 __global__ void __launch_bounds__(kBlock) k_threshold(uint32_t m, const uint32_t* __restrict__ rp,
                                                       const uint32_t* __restrict__ src,
                                                       const int32_t* __restrict__ adj, const uint32_t* __restrict__ cn,
-                                                      double eps, uint8_t* __restrict__ flag) {
+                                                      double eps, const QueryParams* __restrict__ prm,
+                                                      uint8_t* __restrict__ flag) {
+  if (prm) eps = prm->eps;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
     const uint32_t u = src[k], v = (uint32_t)adj[k];
@@ -65,7 +79,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Device bytes of one query slot for a graph of n vertices, m entries.
 size_t slot_bytes(uint64_t n, uint64_t m) {
-  return align256(std::max<uint64_t>(m, 1)) + 5 * align256(n * 4 + 4) + align256(n + 1) + align256(kCntSlots * 8);
+  return align256(std::max<uint64_t>(m, 1)) + 5 * align256(n * 4 + 4) + align256(n + 1) + align256(kCntSlots * 8) +
+         align256(sizeof(QueryParams));
 }
 
 struct Slot {
@@ -77,6 +92,9 @@ struct Slot {
   uint64_t* h_cnt = nullptr;    // pinned
   hipEvent_t start = nullptr, done = nullptr;
   double t_submit = 0;
+  QueryParams* d_params = nullptr;  // device
+  QueryParams* h_params = nullptr;  // pinned
+  hipGraphExec_t exec = nullptr;
 };
 
 struct Graph {
@@ -122,6 +140,7 @@ struct sgc_server {
   std::deque<sgc::Query> pending;
   std::unordered_map<uint64_t, sgc::Result> done;
   std::unordered_map<uint64_t, uint32_t> inflight;  // ticket -> graph
+  bool graph_launch = true;
 };
 
 namespace sgc {
@@ -129,6 +148,8 @@ namespace {
 
 void free_graph(Graph* G) {
   for (Slot& s : G->slots) {
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.h_params) (void)hipHostFree(s.h_params);
     if (s.h_labels) (void)hipHostFree(s.h_labels);
     if (s.h_cnt) (void)hipHostFree(s.h_cnt);
     if (s.start) (void)hipEventDestroy(s.start);
@@ -140,23 +161,66 @@ void free_graph(Graph* G) {
   delete G;
 }
 
+// The work of one query on stream st: eager (p given) or reading eps and mu
+// from the slot's parameter block (p null, for capture).
+void enqueue_query(const Graph* G, const Slot& sl, const ScanParams* p, hipStream_t st) {
+  const sgc_dgraph* g = G->dg;
+  const ScanParams pv = p ? *p : ScanParams{0.0, 0};
+  const QueryParams* prm = p ? nullptr : sl.d_params;
+  const int32_t* mu_dev = p ? nullptr : &sl.d_params->mu;
+  k_query_init<<<grid_for(g->n, kBlock), kBlock, 0, st>>>(g->n, sl.pb.sd, sl.pb.lo, sl.pb.hi, sl.pb.cnt);
+  if (g->m)
+    k_threshold<<<grid_for(g->m, kBlock), kBlock, 0, st>>>(g->m, g->row_ptr, g->src, g->adj, G->cn, pv.eps, prm,
+                                                            sl.pb.flag);
+  launch_core_count(g, sl.pb, st);
+  launch_core_mark(g, sl.pb, pv, st, mu_dev);
+  launch_union(g, sl.pb, st);
+  launch_flatten(g, sl.pb, st);
+  launch_border(g, sl.pb, st);
+  launch_classify_edges(g, sl.pb, st);
+  launch_finalize(g, sl.pb, pv, st, mu_dev);
+  launch_count_similar(g, sl.pb, st);
+  (void)hipMemcpyAsync(sl.h_labels, sl.pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(sl.h_cnt, sl.pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, st);
+}
+
+// Captures the slot's query work into an executable graph.
+int capture_slot(sgc_server* S, const Graph* G, Slot& sl) {
+  hipStream_t st = S->streams[sl.stream];
+  SGC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueue_query(G, sl, nullptr, st);
+  const hipError_t le = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(st, &graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(SGC_EDEVICE, "capture of the query graph failed: %s",
+                hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  const hipError_t ie = hipGraphInstantiate(&sl.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(SGC_EDEVICE, "query graph instantiation failed: %s", hipGetErrorString(ie));
+  return SGC_OK;
+}
+
 // Launches query q into free slot s of graph G.
 int launch_query(sgc_server* S, Graph* G, Slot& sl, const Query& q) {
   hipStream_t st = S->streams[sl.stream];
-  const sgc_dgraph* g = G->dg;
-  const ScanParams p{q.eps, q.mu};
   SGC_HIP(hipEventRecord(sl.start, st));
-  k_query_init<<<grid_for(g->n, kBlock), kBlock, 0, st>>>(g->n, sl.pb.sd, sl.pb.lo, sl.pb.hi, sl.pb.cnt);
-  if (g->m)
-    k_threshold<<<grid_for(g->m, kBlock), kBlock, 0, st>>>(g->m, g->row_ptr, g->src, g->adj, G->cn, q.eps,
-                                                            sl.pb.flag);
-  launch_core(g, sl.pb, p, st);
-  launch_cluster(g, sl.pb, st);
-  launch_border(g, sl.pb, st);
-  launch_classify(g, sl.pb, p, st);
-  SGC_HIP(hipGetLastError());
-  SGC_HIP(hipMemcpyAsync(sl.h_labels, sl.pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, st));
-  SGC_HIP(hipMemcpyAsync(sl.h_cnt, sl.pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, st));
+  if (S->graph_launch) {
+    if (!sl.exec) {
+      const int rc = capture_slot(S, G, sl);
+      if (rc) return rc;
+    }
+    sl.h_params->eps = q.eps;
+    sl.h_params->mu = q.mu;
+    SGC_HIP(hipMemcpyAsync(sl.d_params, sl.h_params, sizeof(QueryParams), hipMemcpyHostToDevice, st));
+    SGC_HIP(hipGraphLaunch(sl.exec, st));
+  } else {
+    const ScanParams p{q.eps, q.mu};
+    enqueue_query(G, sl, &p, st);
+    SGC_HIP(hipGetLastError());
+  }
   SGC_HIP(hipEventRecord(sl.done, st));
   sl.busy = true;
   sl.ticket = q.ticket;
@@ -346,10 +410,13 @@ extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uin
     sl.pb.core = (uint8_t*)p;
     p += align256((size_t)n + 1);
     sl.pb.cnt = (uint64_t*)p;
+    p += align256(kCntSlots * 8);
+    sl.d_params = (QueryParams*)p;
     sl.stream = S->next_stream;
     S->next_stream = (S->next_stream + 1) % kStreams;
     if (hipHostMalloc((void**)&sl.h_labels, std::max<size_t>(n, 1) * 4) != hipSuccess ||
         hipHostMalloc((void**)&sl.h_cnt, kCntSlots * 8) != hipSuccess ||
+        hipHostMalloc((void**)&sl.h_params, sizeof(QueryParams)) != hipSuccess ||
         hipEventCreate(&sl.start) != hipSuccess || hipEventCreate(&sl.done) != hipSuccess) {
       (void)hipGetLastError();
       free_graph(G);
@@ -383,6 +450,13 @@ extern "C" int sgc_server_remove_graph(sgc_server* S, uint32_t graph_id) {
 }
 
 extern "C" uint64_t sgc_server_bytes_in_use(const sgc_server* S) { return S ? S->used : 0; }
+
+extern "C" int sgc_server_set_graph_launch(sgc_server* S, int enable) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  S->graph_launch = enable != 0;
+  return SGC_OK;
+}
 
 extern "C" int sgc_server_submit(sgc_server* S, uint32_t graph_id, double eps, int32_t mu, uint64_t* ticket) {
   clear_error();

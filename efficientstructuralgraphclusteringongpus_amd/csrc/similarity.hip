@@ -8,8 +8,8 @@
 // 2. exclusive scan of the owned-edge counts, scatter of the owned entries
 //    into `ulist` grouped by owner.
 // 3. k_plan: each owner's edges are cut into units of bounded work.
-//    Owners with d(b) <= kT1MaxDeg go to the wave tier, the rest to the
-//    workgroup tier.
+//    Owners with d(b) <= the tuned cut-off (at most kT1MaxDeg) go to the
+//    small tier, the rest to the workgroup tier.
 // 4. Counting (k_sim_units): the owner's row N(b) is put into LDS and the
 //    short rows N(a) of the unit's edges are streamed through it in claims
 //    that every wave of the workgroup takes dynamically (up to 16 coalesced
@@ -33,12 +33,9 @@ namespace {
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-constexpr uint32_t kT1MaxDeg = 1024;
+constexpr uint32_t kT1MaxDeg = 1024;  // small-tier tables are sized for this
 constexpr uint32_t kT1MaxEdges = 256;
-constexpr double kT1UnitWork = 65536.0;
-
 constexpr uint32_t kT2MaxEdges = 512;
-constexpr double kT2UnitWork = 1048576.0;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t shift) {
   return (key * 0x9E3779B1u) >> shift;
@@ -238,7 +235,7 @@ __device__ __forceinline__ uint32_t edges_per_unit(uint32_t cnt, uint64_t work, 
 }
 
 __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __restrict__ rb,
-                                                 const uint32_t* __restrict__ re,
+                                                 const uint32_t* __restrict__ re, SimTuning tune,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
                                                  const uint32_t* __restrict__ ustart, uint2* __restrict__ units1,
@@ -253,11 +250,11 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       if (owned) {
         start = ustart[x];
         const uint32_t d = re[x] - rb[x];
-        if (d <= kT1MaxDeg) {
-          per = edges_per_unit(owned, work_u[x], kT1UnitWork, kT1MaxEdges);
+        if (d <= tune.small_max_degree) {
+          per = edges_per_unit(owned, work_u[x], tune.small_unit_work, kT1MaxEdges);
           c1 = (owned + per - 1) / per;
         } else {
-          per = edges_per_unit(owned, work_u[x], kT2UnitWork, kT2MaxEdges);
+          per = edges_per_unit(owned, work_u[x], tune.large_unit_work, kT2MaxEdges);
           c2 = (owned + per - 1) / per;
         }
       }
@@ -682,8 +679,10 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m == 0) return;
   exclusive_scan(b.cnt_u, b.ustart, g->n, b.scan_tmp, s);
   k_scatter<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.cnt_u, b.ustart, b.fill, b.ulist);
-  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, b.cnt_u, b.work_u, b.ustart,
-                                                   b.units1, b.units2, b.cnt);
+  SimTuning t = g->tuning;
+  if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u,
+                                                   b.ustart, b.units1, b.units2, b.cnt);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,

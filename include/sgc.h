@@ -152,6 +152,25 @@ typedef struct sgc_dgraph sgc_dgraph;
  * SGC_ERANGE - use sgc_scan_ooc for larger graphs). */
 int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out);
 void sgc_dgraph_free(sgc_dgraph* dg);
+
+/* Planner knobs of the tiered similarity kernels for one resident graph.  No
+ * counterpart in the reference, whose launch shape is fixed (one warp per
+ * edge, TOTALTHDCOUNT threads, GPUSCAN++/GPUScan.cu:276-367).  Results do not
+ * depend on them, only the time. */
+typedef struct {
+  uint32_t small_max_degree; /* owners of degree <= this take the small tier (1..1024; default 1024) */
+  double small_unit_work;    /* short-row elements per small-tier work unit (default 65536) */
+  double large_unit_work;    /* per workgroup-tier work unit (default 1048576) */
+  double ms_similarity;      /* sgc_dgraph_autotune: similarity time with these knobs */
+  double ms_default;         /* sgc_dgraph_autotune: similarity time with the defaults */
+  uint32_t trials;           /* sgc_dgraph_autotune: passes timed */
+} sgc_tuning;
+int sgc_dgraph_get_tuning(const sgc_dgraph* dg, sgc_tuning* t);
+int sgc_dgraph_set_tuning(sgc_dgraph* dg, const sgc_tuning* t);
+/* Times the similarity phase of dg at eps under candidate knobs (coordinate
+ * descent over large_unit_work, small_max_degree, small_unit_work; best of
+ * two passes each), keeps the fastest on dg and reports it. */
+int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* dg, double eps, sgc_tuning* best);
 uint64_t sgc_dgraph_n(const sgc_dgraph* dg);
 uint64_t sgc_dgraph_m(const sgc_dgraph* dg);
 
@@ -264,6 +283,10 @@ void sgc_server_destroy(sgc_server* srv);
 /* Device bytes a graph takes in an engine (CSR, cn[], query slots). */
 uint64_t sgc_server_graph_bytes(const sgc_host_graph* g, uint32_t slots_per_graph);
 uint64_t sgc_server_bytes_in_use(const sgc_server* srv);
+/* Launch mode of the query slots: 1 (default) = each slot's work is captured
+ * once into a hipGraph and replayed per query with eps/mu read from device
+ * memory; 0 = eager kernel launches. */
+int sgc_server_set_graph_launch(sgc_server* srv, int enable);
 /* Admission: uploads g and computes every edge's common-neighbour count once
  * (precompute_ms: that pass).  SGC_ENOMEM if the budget cannot hold it; the
  * caller may remove graphs and retry. */

@@ -127,3 +127,31 @@ def test_edgeless_graph_served(ctx):
         t = srv.submit(gid, 0.5, 2)
         labels, st = drain(srv, {t: 3})[t]
         assert list(labels) == [-2, -2, -2] and st.outliers == 3
+
+
+def test_graph_replays_match_eager_launches(ctx, graphs):
+    """One slot, so every query replays the same captured hipGraph; eager work
+    and synchronisation run between the replays and every replay is checked
+    against the eager launch mode and a one-shot pass."""
+    g = graphs[1]
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        params = [(0.2, 2), (0.5, 5), (0.3, 3), (0.9, 2), (0.3, 3), (0.1, 10)]
+        results = {}
+        for mode in (True, False):
+            with sgc.Server(ctx, 4 << 30, slots_per_graph=1) as srv:
+                srv.set_graph_launch(mode)
+                gid, _ = srv.add_graph(g)
+                for eps, mu in params:
+                    say("replay", mode, eps, mu)
+                    t = srv.submit(gid, eps, mu)
+                    labels, st = drain(srv, {t: g.n})[t]
+                    ref = one_shot(ctx, dg, eps, mu)  # eager work between replays
+                    ctx.sync()
+                    assert np.array_equal(labels, ref.labels), (mode, eps, mu)
+                    assert st.cores == ref.stats.cores and st.similar_edges == ref.stats.similar_edges
+                    results[(mode, eps, mu)] = labels
+        for eps, mu in params:
+            assert np.array_equal(results[(True, eps, mu)], results[(False, eps, mu)])
+    finally:
+        dg.close()

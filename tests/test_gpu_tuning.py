@@ -1,0 +1,63 @@
+"""Autotuner of the similarity planner: any knob setting gives identical
+results; the tuned setting is one of the candidates and no slower than the
+defaults as measured (needs an MI355X)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import efficientstructuralgraphclusteringongpus_amd as sgc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.mark.parametrize("knobs", [(128, 16384.0, 262144.0), (512, 262144.0, 4194304.0), (1024, 4096.0, 65536.0),
+                                   (1, 64.0, 64.0)])
+def test_any_knobs_give_identical_results(ctx, knobs):
+    say("knobs", knobs)
+    g = sgc.HostGraph.rmat(16, 16, seed=21)
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        ref = sgc.scan(ctx, dg, 0.3, 3, want_sim=True)
+        dg.tuning = sgc.Tuning(*knobs)
+        t = dg.tuning
+        assert (t.small_max_degree, t.small_unit_work, t.large_unit_work) == knobs
+        r = sgc.scan(ctx, dg, 0.3, 3, want_sim=True)
+        assert np.array_equal(r.sim, ref.sim) and np.array_equal(r.labels, ref.labels)
+    finally:
+        dg.close()
+
+
+def test_autotune_keeps_a_candidate_and_results(ctx):
+    say("autotune")
+    g = sgc.HostGraph.planted(n=200_000, communities=2_000, hubs=16, hub_fraction=0.05, seed=4)
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        ref = sgc.scan(ctx, dg, 0.5, 4, want_sim=True)
+        best = dg.autotune(0.5)
+        say("tuned", best)
+        assert best.small_max_degree in (128, 256, 512, 1024)
+        assert best.small_unit_work in (16384.0, 65536.0, 262144.0)
+        assert best.large_unit_work in (262144.0, 1048576.0, 4194304.0)
+        assert best.trials >= 10 and best.ms_similarity <= best.ms_default
+        t = dg.tuning
+        assert (t.small_max_degree, t.large_unit_work) == (best.small_max_degree, best.large_unit_work)
+        r = sgc.scan(ctx, dg, 0.5, 4, want_sim=True)
+        assert np.array_equal(r.sim, ref.sim) and np.array_equal(r.labels, ref.labels)
+    finally:
+        dg.close()
+
+
+def test_tuning_rejects_out_of_range_knobs(ctx):
+    g = sgc.HostGraph.rmat(10, 8, seed=1)
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        for bad in ((0, 65536.0, 1048576.0), (2048, 65536.0, 1048576.0), (512, 1.0, 1048576.0)):
+            with pytest.raises(sgc.SGCError):
+                dg.tuning = sgc.Tuning(*bad)
+    finally:
+        dg.close()
