@@ -24,7 +24,7 @@ import numpy as np
 __all__ = [
     "SGCError", "HostGraph", "Context", "DeviceGraph", "ScanStats", "ScanResult",
     "scan", "scan_device", "scan_ooc", "ooc_min_budget", "OocStats", "Server", "QueryStats", "Tuning",
-    "copy_bench", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
+    "copy_bench", "write_result", "read_result", "ROLE_NAMES", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
 ]
 
 HUB = -1
@@ -213,6 +213,8 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_dgraph_m", C.c_uint64, [_VP])
     _sig(lib, "sgc_scan", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
     _sig(lib, "sgc_ooc_min_budget", C.c_int, [_VP, C.c_double, _P(C.c_uint64)])
+    _sig(lib, "sgc_write_result", C.c_int,
+         [C.c_char_p, C.c_uint64, _VP, _VP, C.c_double, C.c_int32, _P(_Stats), C.c_int])
     _sig(lib, "sgc_dgraph_get_tuning", C.c_int, [_VP, _P(_Tuning)])
     _sig(lib, "sgc_dgraph_set_tuning", C.c_int, [_VP, _P(_Tuning)])
     _sig(lib, "sgc_dgraph_autotune", C.c_int, [_VP, _VP, C.c_double, _P(_Tuning)])
@@ -586,6 +588,37 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+ROLE_NAMES = ("core", "border", "hub", "outlier")
+
+
+def write_result(out_dir: str, result: ScanResult, eps: float, mu: int, text: bool = False) -> None:
+    """labels.bin / core.bin / role.bin / summary.txt (and labels.txt) in out_dir."""
+    lib = load_library()
+    labels = np.ascontiguousarray(result.labels, dtype=np.int32)
+    core = np.ascontiguousarray(result.core, dtype=np.uint8)
+    st = _Stats()
+    for f in fields(ScanStats):
+        setattr(st, f.name, getattr(result.stats, f.name))
+    _check(lib.sgc_write_result(out_dir.encode(), len(labels), _ptr(labels), _ptr(core), float(eps), int(mu),
+                                C.byref(st), 1 if text else 0))
+
+
+def read_result(out_dir: str) -> dict:
+    """The arrays and summary fields sgc_write_result wrote (no pickle: raw
+    little-endian arrays and "key: value" text)."""
+    out = {
+        "labels": np.fromfile(os.path.join(out_dir, "labels.bin"), dtype="<i4"),
+        "core": np.fromfile(os.path.join(out_dir, "core.bin"), dtype=np.uint8),
+        "role": np.fromfile(os.path.join(out_dir, "role.bin"), dtype=np.uint8),
+    }
+    summary = {}
+    for line in open(os.path.join(out_dir, "summary.txt")):
+        k, _, v = line.rstrip("\n").partition(": ")
+        summary[k] = v
+    out["summary"] = summary
+    return out
 
 
 def GPUScan(folder: str, eps: float, miu: int) -> np.ndarray:

@@ -26,7 +26,7 @@ HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["graph_host.cpp", "ooc_host.cpp"]
+HOST_SOURCES = ["graph_host.cpp", "ooc_host.cpp", "result_io.cpp"]
 HIP_SOURCES = ["kernels.hip", "similarity.hip", "device_api.hip", "ooc.hip", "serve.hip", "batch.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, "-Wall", "-Wno-unused-function"]

@@ -6,6 +6,8 @@
 // "node number and edge number", "core number", "Cluster Number").
 // Options beyond the reference:
 //   --out <file>      write labels as int32[n] followed by core flags uint8[n]
+//   --out-dir <dir>   write labels.bin, core.bin, role.bin and summary.txt
+//   --text            with --out-dir, also labels.txt (one vertex per line)
 //   --device <id>     HIP device (default 0)
 //   --check-symmetry  also verify that every (u,v) has (v,u)
 #include <stdio.h>
@@ -31,11 +33,17 @@ int main(int argc, char* argv[]) {
   const double eps = atof(argv[2]);
   const int miu = atoi(argv[3]);
   const char* out_path = nullptr;
+  const char* out_dir = nullptr;
+  int text = 0;
   int device = 0;
   int check_symmetry = 0;
   for (int i = 4; i < argc; ++i) {
     if (!strcmp(argv[i], "--out") && i + 1 < argc)
       out_path = argv[++i];
+    else if (!strcmp(argv[i], "--out-dir") && i + 1 < argc)
+      out_dir = argv[++i];
+    else if (!strcmp(argv[i], "--text"))
+      text = 1;
     else if (!strcmp(argv[i], "--device") && i + 1 < argc)
       device = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--check-symmetry"))
@@ -107,6 +115,10 @@ int main(int argc, char* argv[]) {
       return 1;
     }
     fclose(f);
+  }
+  if (out_dir && sgc_write_result(out_dir, n, labels.data(), core.data(), eps, miu, &st, text ? SGC_WRITE_TEXT : 0)) {
+    fprintf(stderr, "%s\n", sgc_last_error());
+    return 1;
   }
   sgc_dgraph_free(dg);
   sgc_ctx_destroy(ctx);

@@ -305,6 +305,23 @@ uint32_t sgc_server_pending(const sgc_server* srv);
  * is then released.  SGC_EBUSY while it is queued or running. */
 int sgc_server_result(sgc_server* srv, uint64_t ticket, int32_t* labels, uint64_t capacity, sgc_query_stats* stats);
 
+/* ------------------------------------------------------------------------ */
+/* Result files                                                              */
+/* ------------------------------------------------------------------------ */
+
+#define SGC_WRITE_TEXT 1
+
+/* Writes a run's per-vertex result into directory `dir` (created if needed):
+ * labels.bin int32[n] (cluster id = min core id, -1 hub, -2 outlier),
+ * core.bin uint8[n], role.bin uint8[n] (0 core, 1 border, 2 hub, 3 outlier),
+ * with SGC_WRITE_TEXT also labels.txt ("vertex label role" per line), and
+ * summary.txt: the reference's "core number: %d" and "Cluster Number: %d"
+ * lines (GPUSCAN++/GPUScan.cu:914, main.cpp:111) followed by eps, mu, role
+ * counts, optional pass statistics (st may be NULL) and an FNV-1a digest of
+ * labels.bin.  The reference writes no per-vertex output. */
+int sgc_write_result(const char* dir, uint64_t n, const int32_t* labels, const uint8_t* core, double eps, int32_t mu,
+                     const sgc_stats* st, int flags);
+
 /* Reference-layout graph (GPUSCAN++/Graph.h:13-20). */
 typedef struct {
   long n;

@@ -279,5 +279,12 @@ def test_cli_matches_reference_lines(tmp_path):
     raw = open(out, "rb").read()
     assert np.array_equal(np.frombuffer(raw[: 4 * g.n], dtype=np.int32), labels)
     assert np.array_equal(np.frombuffer(raw[4 * g.n:], dtype=np.uint8), core)
+    res_dir = str(tmp_path / "res")
+    proc = subprocess.run([cli, folder, "0.5", "3", "--out-dir", res_dir, "--text"], capture_output=True, text=True,
+                          timeout=120)
+    assert proc.returncode == 0, proc.stderr
+    back = sgc.read_result(res_dir)
+    assert np.array_equal(back["labels"], labels) and np.array_equal(back["core"], core)
+    assert back["summary"]["Cluster Number"] == str(st["clusters"])
     bad = subprocess.run([cli, folder], capture_output=True, text=True)
     assert bad.returncode == 1 and "parameter eps and miu" in bad.stderr
