@@ -181,6 +181,96 @@ def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
     return out
 
 
+def planted_bench(sgc, ctx, args):
+    """Planted-partition workload: 2M vertices, 20,000 communities with
+    Zipf(1.5) sizes, p_in 0.05 capped at 200 expected intra-degree, 8
+    inter-community edges per vertex, 64 hubs each joined to 5% of the
+    vertices; eps 0.5, mu 4.  The planner knobs are picked by the autotuner
+    first; passes are timed with the defaults and with the tuned knobs."""
+    t0 = time.perf_counter()
+    g = sgc.HostGraph.planted(n=2_000_000, communities=20_000, zipf=1.5, p_in=0.05, max_intra_degree=200.0,
+                              inter_per_vertex=8, hubs=64, hub_fraction=0.05, seed=args.seed)
+    E = g.m // 2
+    log(f"planted: n={g.n} E={E} max_deg={g.max_degree} generated in {time.perf_counter() - t0:.1f} s")
+    dg = sgc.DeviceGraph(ctx, g)
+
+    def timed(k):
+        for _ in range(args.warmup):
+            sgc.scan_device(ctx, dg, 0.5, 4)
+        ctx.sync()
+        t = time.perf_counter()
+        sts = [sgc.scan_device(ctx, dg, 0.5, 4) for _ in range(k)]
+        ctx.sync()
+        return (time.perf_counter() - t) / k, sts[-1]
+
+    default_s, st0 = timed(args.steps)
+    best = dg.autotune(0.5)
+    tuned_s, st1 = timed(args.steps)
+    ref = sgc.scan(ctx, dg, 0.5, 4)
+    dg.tuning = sgc.Tuning()
+    again = sgc.scan(ctx, dg, 0.5, 4)
+    same = bool((ref.labels == again.labels).all())
+    dg.close()
+    out = {
+        "metric": "similarity-edges/s (GTEPS) on the planted-partition graph",
+        "value": round(E / tuned_s / 1e9, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(tuned_s * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "planted partition n=2M, 20k Zipf(1.5) communities, 64 hubs x 5%, eps 0.5 mu 4",
+                   "n": g.n, "edges": E, "max_degree": g.max_degree},
+        "default_ms_per_step": round(default_s * 1e3, 3),
+        "tuning": {"small_max_degree": best.small_max_degree, "small_unit_work": best.small_unit_work,
+                   "large_unit_work": best.large_unit_work, "similarity_ms_tuned": round(best.ms_similarity, 3),
+                   "similarity_ms_default": round(best.ms_default, 3), "trials": best.trials},
+        "labels_identical_default_vs_tuned": same,
+        "stats": {"u_edges": st1.u_edges, "similar_edges": st1.similar_edges, "cores": st1.cores,
+                  "clusters": st1.clusters, "hubs": st1.hubs, "outliers": st1.outliers},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def ooc_bench(sgc, ctx, args):
+    """Out-of-core run: R-MAT scale/edge factor from the arguments, kept in
+    host memory, clustered within --budget-gib of device memory; checked
+    against the resident path when the graph fits one."""
+    t0 = time.perf_counter()
+    g = sgc.HostGraph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+    E = g.m // 2
+    log(f"ooc: R-MAT-{args.scale} ef{args.edge_factor:g} n={g.n} m={g.m} generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    eps, mu = float(args.eps.split(",")[0]), int(args.mu.split(",")[0])
+    budget = int(args.budget_gib * (1 << 30))
+    need = sgc.ooc_min_budget(g, eps)
+    log(f"ooc: budget {budget} bytes, minimum {need}")
+    t0 = time.perf_counter()
+    r, info = sgc.scan_ooc(ctx, g, eps, mu, budget)
+    wall = time.perf_counter() - t0
+    check = None
+    if g.m < (1 << 32):
+        dg = sgc.DeviceGraph(ctx, g)
+        t1 = time.perf_counter()
+        ref = sgc.scan(ctx, dg, eps, mu)
+        resident_s = time.perf_counter() - t1
+        dg.close()
+        check = {"labels_identical": bool((ref.labels == r.labels).all()),
+                 "core_identical": bool((ref.core == r.core).all()), "resident_pass_s": round(resident_s, 4)}
+    out = {
+        "metric": "similarity-edges/s (GTEPS), out-of-core", "value": round(E / wall / 1e9, 4), "unit": "GTEPS",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor:g}, eps {eps} mu {mu}, host-resident, "
+                               f"device budget {budget} bytes", "n": g.n, "edges": E},
+        "ooc": {"device_bytes": info.device_bytes, "similarity_chips": info.similarity_chips,
+                "row_chips": info.row_chips, "bytes_h2d": info.bytes_h2d, "bytes_d2h": info.bytes_d2h,
+                "ms_host": round(info.ms_host, 1), "ms_phases": round(info.ms_phases, 1),
+                "similarity_device_ms": round(r.stats.ms_similarity, 1)},
+        "check_vs_resident": check,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def measured_traffic(args):
     """HBM bytes per launch of the workgroup-tier similarity kernel on the
     default workload, from the committed rocprofv3 counter summary
@@ -206,6 +296,10 @@ def main() -> int:
     ap.add_argument("--eps", default="0.2,0.4,0.6,0.8")
     ap.add_argument("--mu", default="2,5,10")
     ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget; 0 skips it")
+    ap.add_argument("--workload", choices=["rmat", "planted", "ooc"], default="rmat",
+                    help="rmat: the headline R-MAT run (default); planted: planted partition with the autotuner; "
+                         "ooc: out-of-core run within --budget-gib")
+    ap.add_argument("--budget-gib", type=float, default=1.5, help="device budget of --workload ooc")
     ap.add_argument("--serve-queries", type=int, default=2000,
                     help="queries of the serving run (32 resident graphs); 0 skips it")
     ap.add_argument("--copy-gib", type=float, default=1.0,
@@ -219,6 +313,10 @@ def main() -> int:
         return 2
 
     import efficientstructuralgraphclusteringongpus_amd as sgc
+
+    if args.workload != "rmat":
+        with sgc.Context(0) as ctx:
+            return (planted_bench if args.workload == "planted" else ooc_bench)(sgc, ctx, args)
 
     eps_list = [float(x) for x in args.eps.split(",")]
     mu_list = [int(x) for x in args.mu.split(",")]

@@ -73,6 +73,16 @@ struct SimTuning {
 };
 }  // namespace sgc
 
+namespace sgc {
+// Out-of-core chip of two row blocks [lo0, hi0) and [lo1, hi1) (equal for a
+// chip of one block): the prep kernel keeps only owned edges whose endpoints
+// lie one in each block.
+struct PairFilter {
+  uint32_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+  bool on = false;
+};
+}  // namespace sgc
+
 struct sgc_dgraph {
   sgc_ctx* ctx = nullptr;
   uint32_t n = 0;
@@ -89,6 +99,7 @@ struct sgc_dgraph {
   const uint32_t* row_begin = nullptr;
   const uint32_t* row_end = nullptr;
   sgc::SimTuning tuning;
+  sgc::PairFilter pair;
 };
 
 namespace sgc {

@@ -5,15 +5,17 @@
 // Per-vertex state stays resident (degrees, the chip row table, similar-degree
 // counts, union-find forest, labels, classify bounds, planner arrays: about
 // 57 bytes per vertex); the rest of the budget holds one chip at a time.
-//  * Similarity: a chip is a range of owner rows plus every short row its
-//    undecided edges read, stored once.  The chip's row table maps those
-//    rows into the chip; absent rows keep only their degree (a range of the
-//    right length that no kernel reads).  The in-memory prep, planner and
-//    similarity kernels run on the chip unchanged; mirror entries land where
-//    the short row sits in the chip.  Final flags go back into a pinned
-//    whole-graph flag array.  The next chip is assembled on the host while
-//    the device works on the current one, and the previous chip's flags are
-//    merged meanwhile.
+//  * Similarity: rows are cut into P blocks of at most half a chip.  Chip
+//    (i, j), j >= i, holds blocks i and j and decides exactly the undecided
+//    edges with one endpoint in each (the prep kernel's pair filter marks the
+//    other owned edges kFlagUSkip); every edge has both of its rows in exactly
+//    one chip.  The resident prep, planner and similarity kernels run on the
+//    chip unchanged through its row table (absent rows keep only their
+//    degree).  Block i stays on the device while j advances, so the host
+//    sends about m * (1 + P / 2) entries rather than a copy of every short row
+//    per owner.  Final flags return to a pinned whole-graph flag array; the
+//    next block is staged and the previous chip's flags are merged on the
+//    host while the device works.
 //  * Core check, union-find, border and classify: chips of whole rows with
 //    their flags, one pass per phase (each phase needs the previous one's
 //    global result); when all rows fit in one chip it is uploaded once.
@@ -50,18 +52,6 @@ __global__ void __launch_bounds__(kBlock) k_rows_range(uint32_t r0, uint32_t nr,
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_rows_list(uint32_t cnt, const uint32_t* __restrict__ ids,
-                                                      const uint32_t* __restrict__ starts,
-                                                      const uint32_t* __restrict__ deg, uint32_t* __restrict__ rb,
-                                                      uint32_t* __restrict__ re) {
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += stride) {
-    const uint32_t x = ids[i];
-    rb[x] = starts[i];
-    re[x] = starts[i] + deg[x];
-  }
-}
-
 // src[k] = r0 + i for the entries [off[i], off[i+1]) of chip row i; one wave
 // per row.
 __global__ void __launch_bounds__(kBlock) k_fill_src_rows(uint32_t r0, uint32_t nr, const uint32_t* __restrict__ off,
@@ -89,47 +79,28 @@ struct Layout {
 };
 
 // Bytes per entry of the two chip kinds (adj, src, flag, ulist, two unit
-// arrays, short-row ids and starts / adj, src, flag).
-constexpr uint64_t kSimChipBytesPerEntry = 4 + 4 + 1 + 4 + 8 + 8 + 4 + 4;
+// arrays / adj, src, flag).
+constexpr uint64_t kSimChipBytesPerEntry = 4 + 4 + 1 + 4 + 8 + 8;
 constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
-struct OocRun {
-  sgc_ctx* ctx = nullptr;
-  const sgc_host_graph* g = nullptr;
-  uint32_t n = 0;
-  void* dev = nullptr;
-  // resident
-  uint32_t *deg = nullptr, *rb = nullptr, *re = nullptr, *off = nullptr;
-  PassBuffers pb{};
-  // chip region
-  char* chip = nullptr;
-  size_t chip_bytes = 0;
-  // pinned host staging, two of each for overlap
-  int32_t* h_adj[2] = {nullptr, nullptr};
-  uint32_t* h_off[2] = {nullptr, nullptr};
-  uint8_t* h_flag[2] = {nullptr, nullptr};
-  uint64_t* h_cnt[2] = {nullptr, nullptr};
-  uint8_t* flag = nullptr;  // whole-graph flags, pinned
-  hipEvent_t done[2] = {nullptr, nullptr};
-  double bytes_h2d = 0, bytes_d2h = 0, ms_host = 0;
-
-  ~OocRun() {
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 2; ++i) {
-      if (h_adj[i]) (void)hipHostFree(h_adj[i]);
-      if (h_off[i]) (void)hipHostFree(h_off[i]);
-      if (h_flag[i]) (void)hipHostFree(h_flag[i]);
-      if (h_cnt[i]) (void)hipHostFree(h_cnt[i]);
-      if (done[i]) (void)hipEventDestroy(done[i]);
-    }
-    if (flag) (void)hipHostFree(flag);
-    if (dev) (void)hipFree(dev);
-  }
+struct ResidentLayout {
+  size_t deg, rb, re, off_i, off_j, sd, par, lab, lo, hi, core, cntu, work, fill, ust, scan, cnt, bytes;
 };
 
-// Similarity-chip view: the owner entries [0, m_own) are the ones the
-// kernels decide; short rows follow in the same arrays.
+ResidentLayout resident_layout(uint32_t n) {
+  Layout L;
+  ResidentLayout r;
+  r.deg = L.take((size_t)n * 4), r.rb = L.take((size_t)n * 4), r.re = L.take((size_t)n * 4);
+  r.off_i = L.take(((size_t)n + 1) * 4), r.off_j = L.take(((size_t)n + 1) * 4);
+  r.sd = L.take((size_t)n * 4), r.par = L.take((size_t)n * 4), r.lab = L.take((size_t)n * 4);
+  r.lo = L.take((size_t)n * 4), r.hi = L.take((size_t)n * 4), r.core = L.take(n);
+  r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.fill = L.take((size_t)n * 4);
+  r.ust = L.take(((size_t)n + 1) * 4), r.scan = L.take(scan_tmp_words(n) * 4), r.cnt = L.take(kCntSlots * 8);
+  r.bytes = L.bytes;
+  return r;
+}
+
 struct SimChipArrays {
   int32_t* adj;
   uint32_t* src;
@@ -137,25 +108,22 @@ struct SimChipArrays {
   uint32_t* ulist;
   uint2* units1;
   uint2* units2;
-  uint32_t* ids;
-  uint32_t* starts;
 };
 
-SimChipArrays sim_arrays(char* base, uint64_t cap) {
+SimChipArrays sim_arrays(char* base, uint64_t cap, size_t* bytes) {
   Layout L;
   const size_t o_adj = L.take(cap * 4), o_src = L.take(cap * 4), o_flag = L.take(cap),
                o_ulist = L.take(cap * 4), o_u1 = L.take(unit_capacity(cap) * sizeof(uint2)),
-               o_u2 = L.take(unit_capacity(cap) * sizeof(uint2)), o_ids = L.take(cap * 4), o_st = L.take(cap * 4);
-  return SimChipArrays{(int32_t*)(base + o_adj),  (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
-                       (uint32_t*)(base + o_ulist), (uint2*)(base + o_u1),    (uint2*)(base + o_u2),
-                       (uint32_t*)(base + o_ids), (uint32_t*)(base + o_st)};
+               o_u2 = L.take(unit_capacity(cap) * sizeof(uint2));
+  if (bytes) *bytes = L.bytes;
+  return SimChipArrays{(int32_t*)(base + o_adj),    (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
+                       (uint32_t*)(base + o_ulist), (uint2*)(base + o_u1),    (uint2*)(base + o_u2)};
 }
 
 size_t sim_chip_bytes(uint64_t cap) {
-  Layout L;
-  L.take(cap * 4), L.take(cap * 4), L.take(cap), L.take(cap * 4), L.take(unit_capacity(cap) * sizeof(uint2)),
-      L.take(unit_capacity(cap) * sizeof(uint2)), L.take(cap * 4), L.take(cap * 4);
-  return L.bytes;
+  size_t b = 0;
+  sim_arrays(nullptr, cap, &b);
+  return b;
 }
 
 size_t row_chip_bytes(uint64_t cap) {
@@ -164,16 +132,53 @@ size_t row_chip_bytes(uint64_t cap) {
   return L.bytes;
 }
 
-size_t resident_bytes(uint32_t n) {
-  Layout L;
-  for (int i = 0; i < 3; ++i) L.take((size_t)n * 4);  // deg, row table
-  L.take(((size_t)n + 1) * 4);                        // owner offsets
-  for (int i = 0; i < 5; ++i) L.take((size_t)n * 4);  // sd, parent, label, lo, hi
-  L.take(n);                                          // core
-  L.take((size_t)n * 4), L.take((size_t)n * 8), L.take((size_t)n * 4), L.take(((size_t)n + 1) * 4);
-  L.take(scan_tmp_words(n) * 4), L.take(kCntSlots * 8);
-  return L.bytes;
-}
+// Pinned copy of one row block, free again once its upload has run.
+struct Staging {
+  int32_t* adj = nullptr;
+  uint32_t* off = nullptr;
+  hipEvent_t free = nullptr;
+  bool pending = false;  // an upload from it may still be in flight
+};
+
+struct OocRun {
+  sgc_ctx* ctx = nullptr;
+  const sgc_host_graph* g = nullptr;
+  uint32_t n = 0;
+  void* dev = nullptr;
+  uint32_t *deg = nullptr, *rb = nullptr, *re = nullptr, *off_i = nullptr, *off_j = nullptr;
+  PassBuffers pb{};
+  char* chip = nullptr;
+  Staging stage[3];  // block i, and two for the j blocks
+  uint8_t* h_flag[2] = {nullptr, nullptr};
+  uint64_t* h_cnt[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr}, t0[2] = {nullptr, nullptr}, tm[2] = {nullptr, nullptr},
+             t1[2] = {nullptr, nullptr};
+  uint8_t* flag = nullptr;  // whole-graph flags, pinned
+  double bytes_h2d = 0, bytes_d2h = 0, ms_host = 0;
+
+  ~OocRun() {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    for (Staging& st : stage) {
+      if (st.adj) (void)hipHostFree(st.adj);
+      if (st.off) (void)hipHostFree(st.off);
+      if (st.free) (void)hipEventDestroy(st.free);
+    }
+    for (int i = 0; i < 2; ++i) {
+      if (h_flag[i]) (void)hipHostFree(h_flag[i]);
+      if (h_cnt[i]) (void)hipHostFree(h_cnt[i]);
+      for (hipEvent_t e : {done[i], t0[i], tm[i], t1[i]})
+        if (e) (void)hipEventDestroy(e);
+    }
+    if (flag) (void)hipHostFree(flag);
+    if (dev) (void)hipFree(dev);
+  }
+};
+
+// One chip of the similarity phase: blocks i and j (j == i: one block).
+struct ChipDesc {
+  uint64_t r0i, r1i, r0j, r1j;
+  uint64_t mi, mj;
+};
 
 int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels, uint8_t* core_out, uint8_t* sim_out,
             sgc_stats* st, sgc_ooc_stats* oi) {
@@ -183,72 +188,73 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   hipStream_t s = ctx->stream;
   const double t_start = now_ms();
 
-  // ---- resident layout
-  Layout L;
-  const size_t o_deg = L.take((size_t)n * 4), o_rb = L.take((size_t)n * 4), o_re = L.take((size_t)n * 4),
-               o_off = L.take(((size_t)n + 1) * 4), o_sd = L.take((size_t)n * 4), o_par = L.take((size_t)n * 4),
-               o_lab = L.take((size_t)n * 4), o_lo = L.take((size_t)n * 4), o_hi = L.take((size_t)n * 4),
-               o_core = L.take(n), o_cntu = L.take((size_t)n * 4), o_work = L.take((size_t)n * 8),
-               o_fill = L.take((size_t)n * 4), o_ust = L.take(((size_t)n + 1) * 4),
-               o_scan = L.take(scan_tmp_words(n) * 4), o_cnt = L.take(kCntSlots * 8);
-  const size_t resident = L.bytes;  // == resident_bytes(n)
-  if (budget <= resident)
+  // ---- resident layout and plans
+  const ResidentLayout RLay = resident_layout(n);
+  const size_t resident = RLay.bytes;
+  if (budget <= resident + 8192)
     return fail(SGC_ENOMEM, "device budget %llu bytes is below the %llu resident bytes the graph's %u vertices need",
                 (unsigned long long)budget, (unsigned long long)resident, n);
-  const uint64_t room = budget - resident - 4096;
+  const uint64_t room = budget - resident - 8192;
   const uint64_t cap_sim = std::min<uint64_t>(room / kSimChipBytesPerEntry, kMaxChipEntries);
   const uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries);
-
-  // ---- plans
   double t0 = now_ms();
   std::vector<uint32_t> deg(n);
   ooc_degrees(g, deg.data());
-  std::vector<uint64_t> sim_bounds, row_bounds;
-  uint64_t sim_max = 0, row_max = 0;
-  int rc = ooc_plan_similarity(g, deg.data(), eps, cap_sim, &sim_bounds, &sim_max);
+  std::vector<uint64_t> blocks, row_bounds;
+  uint64_t block_max = 0, row_max = 0;
+  int rc = ooc_plan_rows(g, cap_sim / 2, &blocks, &block_max);
   if (rc) return rc;
   rc = ooc_plan_rows(g, cap_row, &row_bounds, &row_max);
   if (rc) return rc;
-  const size_t chip_need = std::max(sim_chip_bytes(std::max<uint64_t>(sim_max, 1)),
-                                    row_chip_bytes(std::max<uint64_t>(row_max, 1)));
+  const uint64_t sim_entries = std::max<uint64_t>(2 * block_max, 1);
+  const size_t chip_need = std::max(sim_chip_bytes(sim_entries), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
   R.ms_host += now_ms() - t0;
 
   // ---- device and pinned host memory
-  R.chip_bytes = chip_need;
   if (hipMalloc(&R.dev, resident + chip_need) != hipSuccess) {
     (void)hipGetLastError();
     return fail(SGC_ENOMEM, "device allocation of %llu bytes failed", (unsigned long long)(resident + chip_need));
   }
   char* base = (char*)R.dev;
-  R.deg = (uint32_t*)(base + o_deg);
-  R.rb = (uint32_t*)(base + o_rb);
-  R.re = (uint32_t*)(base + o_re);
-  R.off = (uint32_t*)(base + o_off);
+  R.deg = (uint32_t*)(base + RLay.deg);
+  R.rb = (uint32_t*)(base + RLay.rb);
+  R.re = (uint32_t*)(base + RLay.re);
+  R.off_i = (uint32_t*)(base + RLay.off_i);
+  R.off_j = (uint32_t*)(base + RLay.off_j);
   PassBuffers& pb = R.pb;
-  pb.sd = (uint32_t*)(base + o_sd);
-  pb.parent = (int32_t*)(base + o_par);
-  pb.label = (int32_t*)(base + o_lab);
-  pb.lo = (int32_t*)(base + o_lo);
-  pb.hi = (int32_t*)(base + o_hi);
-  pb.core = (uint8_t*)(base + o_core);
-  pb.cnt_u = (uint32_t*)(base + o_cntu);
-  pb.work_u = (uint64_t*)(base + o_work);
-  pb.fill = (uint32_t*)(base + o_fill);
-  pb.ustart = (uint32_t*)(base + o_ust);
-  pb.scan_tmp = (uint32_t*)(base + o_scan);
-  pb.cnt = (uint64_t*)(base + o_cnt);
+  pb.sd = (uint32_t*)(base + RLay.sd);
+  pb.parent = (int32_t*)(base + RLay.par);
+  pb.label = (int32_t*)(base + RLay.lab);
+  pb.lo = (int32_t*)(base + RLay.lo);
+  pb.hi = (int32_t*)(base + RLay.hi);
+  pb.core = (uint8_t*)(base + RLay.core);
+  pb.cnt_u = (uint32_t*)(base + RLay.cntu);
+  pb.work_u = (uint64_t*)(base + RLay.work);
+  pb.fill = (uint32_t*)(base + RLay.fill);
+  pb.ustart = (uint32_t*)(base + RLay.ust);
+  pb.scan_tmp = (uint32_t*)(base + RLay.scan);
+  pb.cnt = (uint64_t*)(base + RLay.cnt);
   R.chip = base + resident;
 
-  const uint64_t sim_entries = std::max<uint64_t>(sim_max, 1);
+  const uint64_t stage_entries = std::max<uint64_t>(block_max, 1);
+  for (Staging& sg : R.stage) {
+    if (hipHostMalloc((void**)&sg.adj, stage_entries * 4) != hipSuccess ||
+        hipHostMalloc((void**)&sg.off, ((size_t)n + 1) * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SGC_ENOMEM, "pinned host staging of %llu bytes failed", (unsigned long long)(stage_entries * 4));
+    }
+    SGC_HIP(hipEventCreateWithFlags(&sg.free, hipEventDisableTiming));
+  }
   for (int i = 0; i < 2; ++i) {
-    if (hipHostMalloc((void**)&R.h_adj[i], sim_entries * 4) != hipSuccess ||
-        hipHostMalloc((void**)&R.h_off[i], ((size_t)n + 1) * 4) != hipSuccess ||
-        hipHostMalloc((void**)&R.h_flag[i], sim_entries) != hipSuccess ||
+    if (hipHostMalloc((void**)&R.h_flag[i], sim_entries) != hipSuccess ||
         hipHostMalloc((void**)&R.h_cnt[i], kCntSlots * 8) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(SGC_ENOMEM, "pinned host staging of %llu bytes failed", (unsigned long long)(sim_entries * 5));
+      return fail(SGC_ENOMEM, "pinned host flag staging of %llu bytes failed", (unsigned long long)sim_entries);
     }
     SGC_HIP(hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming));
+    SGC_HIP(hipEventCreate(&R.t0[i]));
+    SGC_HIP(hipEventCreate(&R.tm[i]));
+    SGC_HIP(hipEventCreate(&R.t1[i]));
   }
   if (hipHostMalloc((void**)&R.flag, std::max<uint64_t>(g->m, 1)) != hipSuccess) {
     (void)hipGetLastError();
@@ -261,82 +267,103 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   uint64_t totals[kCntSlots] = {0};
   double ms_sim = 0, ms_sim_large = 0;
 
-  // ---- similarity chips
-  const size_t nsim = sim_bounds.size() - 1;
-  std::vector<int32_t> stamp(n, -1);
-  OocChip chips[2];
-  auto build = [&](size_t i) {
+  // ---- similarity over chips of block pairs
+  const size_t P = blocks.size() - 1;
+  const SimChipArrays A = sim_arrays(R.chip, sim_entries, nullptr);
+  auto stage_block = [&](Staging& sg, size_t b, uint32_t base_off) -> int {
+    if (sg.pending) SGC_HIP(hipEventSynchronize(sg.free));
+    sg.pending = false;
     const double tb = now_ms();
-    ooc_build_chip(g, deg.data(), eps, sim_bounds[i], sim_bounds[i + 1], stamp.data(), (int32_t)i, R.h_adj[i & 1],
-                   R.h_off[i & 1], &chips[i & 1]);
-    R.ms_host += now_ms() - tb;
-  };
-  auto merge = [&](size_t i) -> int {
-    SGC_HIP(hipEventSynchronize(R.done[i & 1]));
-    const double tb = now_ms();
-    ooc_merge_flags(g, deg.data(), chips[i & 1], R.h_flag[i & 1], R.flag);
-    for (int c = 0; c < kCntSlots; ++c) totals[c] += R.h_cnt[i & 1][c];
+    ooc_copy_rows(g, blocks[b], blocks[b + 1], base_off, sg.adj, sg.off);
     R.ms_host += now_ms() - tb;
     return SGC_OK;
   };
-  const SimChipArrays A = sim_arrays(R.chip, sim_entries);
-  if (g->m) build(0);
-  for (size_t i = 0; g->m && i < nsim; ++i) {
-    const OocChip& c = chips[i & 1];
-    const uint32_t nr = (uint32_t)(c.r1 - c.r0), nshort = (uint32_t)c.ids.size();
-    // the device may still read the previous chip: stream order serialises
-    SGC_HIP(hipMemcpyAsync(A.adj, R.h_adj[i & 1], c.m_chip * 4, hipMemcpyHostToDevice, s));
-    SGC_HIP(hipMemcpyAsync(R.off, R.h_off[i & 1], ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
-    if (nshort) {
-      SGC_HIP(hipMemcpyAsync(A.ids, c.ids.data(), (size_t)nshort * 4, hipMemcpyHostToDevice, s));
-      SGC_HIP(hipMemcpyAsync(A.starts, c.starts.data(), (size_t)nshort * 4, hipMemcpyHostToDevice, s));
+  auto upload_block = [&](Staging& sg, size_t b, int32_t* dst, uint32_t* off_dev) -> int {
+    const uint64_t m_b = g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]];
+    const uint32_t nr = (uint32_t)(blocks[b + 1] - blocks[b]);
+    if (m_b) SGC_HIP(hipMemcpyAsync(dst, sg.adj, m_b * 4, hipMemcpyHostToDevice, s));
+    SGC_HIP(hipMemcpyAsync(off_dev, sg.off, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
+    SGC_HIP(hipEventRecord(sg.free, s));
+    sg.pending = true;
+    R.bytes_h2d += (double)m_b * 4 + (double)(nr + 1) * 4;
+    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)blocks[b], nr, off_dev, A.src);
+    return SGC_OK;
+  };
+  ChipDesc prev{};
+  size_t chip_no = 0;
+  auto merge = [&](const ChipDesc& c, size_t k) -> int {
+    SGC_HIP(hipEventSynchronize(R.done[k & 1]));
+    const double tb = now_ms();
+    ooc_merge_rows(g, c.r0i, c.r1i, R.h_flag[k & 1], R.flag);
+    if (c.r0j != c.r0i) ooc_merge_rows(g, c.r0j, c.r1j, R.h_flag[k & 1] + c.mi, R.flag);
+    for (int q = 0; q < kCntSlots; ++q) totals[q] += R.h_cnt[k & 1][q];
+    R.ms_host += now_ms() - tb;
+    ms_sim += elapsed(R.t0[k & 1], R.t1[k & 1]);
+    ms_sim_large += elapsed(R.t0[k & 1], R.tm[k & 1]);
+    return SGC_OK;
+  };
+  for (size_t i = 0; g->m && i < P; ++i) {
+    ChipDesc c;
+    c.r0i = blocks[i], c.r1i = blocks[i + 1];
+    c.mi = g->row_ptr[c.r1i] - g->row_ptr[c.r0i];
+    if ((rc = stage_block(R.stage[0], i, 0))) return rc;
+    if ((rc = upload_block(R.stage[0], i, A.adj, R.off_i))) return rc;
+    for (size_t j = i; j < P; ++j) {
+      Staging& sj = R.stage[1 + (j & 1)];
+      if (j > i) {
+        c.r0j = blocks[j], c.r1j = blocks[j + 1];
+        c.mj = g->row_ptr[c.r1j] - g->row_ptr[c.r0j];
+        // block j was staged (offsets based at mi) while chip (i, j-1) ran
+        if ((rc = upload_block(sj, j, A.adj + c.mi, R.off_j))) return rc;
+      } else {
+        c.r0j = c.r0i, c.r1j = c.r1i, c.mj = 0;
+      }
+      const uint32_t nri = (uint32_t)(c.r1i - c.r0i), nrj = (uint32_t)(c.r1j - c.r0j);
+      k_rows_absent<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, R.deg, R.rb, R.re);
+      if (nri) k_rows_range<<<grid_for(nri, kBlock), kBlock, 0, s>>>((uint32_t)c.r0i, nri, R.off_i, R.rb, R.re);
+      if (j > i && nrj)
+        k_rows_range<<<grid_for(nrj, kBlock), kBlock, 0, s>>>((uint32_t)c.r0j, nrj, R.off_j, R.rb, R.re);
+      const uint64_t m_chip = c.mi + c.mj;
+      sgc_dgraph view;
+      view.ctx = ctx;
+      view.n = n;
+      view.m = (uint32_t)m_chip;
+      view.adj = A.adj;
+      view.src = A.src;
+      view.row_begin = R.rb;
+      view.row_end = R.re;
+      view.pair.lo0 = (uint32_t)c.r0i, view.pair.hi0 = (uint32_t)c.r1i;
+      view.pair.lo1 = (uint32_t)c.r0j, view.pair.hi1 = (uint32_t)c.r1j;
+      view.pair.on = true;
+      PassBuffers cb = pb;
+      cb.flag = A.flag;
+      cb.ulist = A.ulist;
+      cb.units1 = A.units1;
+      cb.units2 = A.units2;
+      const size_t k = chip_no;
+      launch_init(&view, cb, s);
+      launch_prep(&view, cb, p, s);
+      launch_plan(&view, cb, s);
+      SGC_HIP(hipEventRecord(R.t0[k & 1], s));
+      launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[k & 1], s);
+      SGC_HIP(hipEventRecord(R.t1[k & 1], s));
+      SGC_HIP(hipGetLastError());
+      if (m_chip) SGC_HIP(hipMemcpyAsync(R.h_flag[k & 1], A.flag, m_chip, hipMemcpyDeviceToHost, s));
+      SGC_HIP(hipMemcpyAsync(R.h_cnt[k & 1], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
+      SGC_HIP(hipEventRecord(R.done[k & 1], s));
+      R.bytes_d2h += (double)m_chip;
+      // while the device works: merge the previous chip, stage the next block
+      if (k > 0 && (rc = merge(prev, k - 1))) return rc;
+      if (j + 1 < P && j + 1 > i) {
+        Staging& nx = R.stage[1 + ((j + 1) & 1)];
+        if ((rc = stage_block(nx, j + 1, (uint32_t)c.mi))) return rc;
+      }
+      prev = c;
+      ++chip_no;
     }
-    R.bytes_h2d += (double)c.m_chip * 4 + (double)(nr + 1) * 4 + (double)nshort * 8;
-    k_rows_absent<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, R.deg, R.rb, R.re);
-    if (nr) {
-      k_rows_range<<<grid_for(nr, kBlock), kBlock, 0, s>>>((uint32_t)c.r0, nr, R.off, R.rb, R.re);
-      k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0, nr, R.off, A.src);
-    }
-    if (nshort) k_rows_list<<<grid_for(nshort, kBlock), kBlock, 0, s>>>(nshort, A.ids, A.starts, R.deg, R.rb, R.re);
-    if (c.m_chip > c.m_own) SGC_HIP(hipMemsetAsync(A.flag + c.m_own, 0xFF, c.m_chip - c.m_own, s));
-    sgc_dgraph view;
-    view.ctx = ctx;
-    view.n = n;
-    view.m = (uint32_t)c.m_own;
-    view.adj = A.adj;
-    view.src = A.src;
-    view.row_begin = R.rb;
-    view.row_end = R.re;
-    PassBuffers cb = pb;
-    cb.flag = A.flag;
-    cb.ulist = A.ulist;
-    cb.units1 = A.units1;
-    cb.units2 = A.units2;
-    launch_init(&view, cb, s);
-    launch_prep(&view, cb, p, s);
-    launch_plan(&view, cb, s);
-    SGC_HIP(hipEventRecord(ctx->ev[1], s));
-    launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, ctx->ev[7], s);
-    SGC_HIP(hipEventRecord(ctx->ev[2], s));
-    SGC_HIP(hipGetLastError());
-    SGC_HIP(hipMemcpyAsync(R.h_flag[i & 1], A.flag, c.m_chip, hipMemcpyDeviceToHost, s));
-    SGC_HIP(hipMemcpyAsync(R.h_cnt[i & 1], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
-    SGC_HIP(hipEventRecord(R.done[i & 1], s));
-    R.bytes_d2h += (double)c.m_chip;
-    // overlap: merge the previous chip and assemble the next one on the host
-    if (i > 0) {
-      rc = merge(i - 1);
-      if (rc) return rc;
-    }
-    if (i + 1 < nsim) build(i + 1);
-    SGC_HIP(hipEventSynchronize(ctx->ev[2]));
-    ms_sim += elapsed(ctx->ev[1], ctx->ev[2]);
-    ms_sim_large += c.m_own ? elapsed(ctx->ev[1], ctx->ev[7]) : 0.0;
   }
-  if (g->m && nsim) {
-    rc = merge(nsim - 1);
-    if (rc) return rc;
-  }
+  if (chip_no && (rc = merge(prev, chip_no - 1))) return rc;
+  const size_t nsim = chip_no;
 
   // ---- core check, union-find, border, classify over row chips
   const double t_phase = now_ms();
@@ -364,16 +391,16 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     v->src = c_src;
     if (loaded == i) return SGC_OK;
     const uint32_t nr = (uint32_t)(r1 - r0);
-    uint32_t* ho = R.h_off[0];
+    uint32_t* ho = R.stage[0].off;
     SGC_HIP(hipStreamSynchronize(s));  // staging and chip buffers are reused
     for (uint32_t k = 0; k <= nr; ++k) ho[k] = (uint32_t)(g->row_ptr[r0 + k] - b0);
-    SGC_HIP(hipMemcpyAsync(R.off, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
+    SGC_HIP(hipMemcpyAsync(R.off_i, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
     if (mc) {
       SGC_HIP(hipMemcpyAsync(c_adj, g->adj + b0, mc * 4, hipMemcpyHostToDevice, s));
       SGC_HIP(hipMemcpyAsync(c_flag, R.flag + b0, mc, hipMemcpyHostToDevice, s));
     }
     R.bytes_h2d += (double)mc * 5 + (double)(nr + 1) * 4;
-    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, R.off, c_src);
+    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, R.off_i, c_src);
     loaded = i;
     return SGC_OK;
   };
@@ -461,16 +488,14 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
 
 extern "C" int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t* bytes) {
   using namespace sgc;
+  (void)eps;  // block chips need the same room at every eps
   clear_error();
   if (!g || !bytes) return fail(SGC_EINVAL, "null argument");
   if (g->n >= (1ull << 31)) return fail(SGC_ERANGE, "n=%llu: vertex ids must fit int32", (unsigned long long)g->n);
-  std::vector<uint32_t> deg(g->n);
-  ooc_degrees(g, deg.data());
-  uint64_t maxd = 0;
-  for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, deg[x]);
-  const uint64_t cost = ooc_max_row_cost(g, deg.data(), eps);
-  const uint64_t chip = std::max<uint64_t>(cost * kSimChipBytesPerEntry, maxd * kRowChipBytesPerEntry);
-  *bytes = resident_bytes((uint32_t)g->n) + 4096 + chip + 4096;
+  uint64_t maxd = 1;
+  for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, g->row_ptr[x + 1] - g->row_ptr[x]);
+  // a chip of two blocks that each hold the longest row
+  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + 2 * maxd * kSimChipBytesPerEntry + 4096;
   return SGC_OK;
 }
 

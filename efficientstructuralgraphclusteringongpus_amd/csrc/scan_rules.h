@@ -17,7 +17,9 @@ namespace sgc {
 // marked in both rows: kFlagUDown in the higher (degree, id) endpoint's row,
 // which owns the computation, kFlagUUp in the other row, which receives the
 // result.  After the similarity phase every flag is kFlagN or kFlagY.
-enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagUUp = 2, kFlagUDown = 3 };
+// kFlagUSkip: an owned undecided edge that an out-of-core chip leaves to the
+// chip holding both of its endpoints' blocks.
+enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagUUp = 2, kFlagUDown = 3, kFlagUSkip = 4 };
 
 // Degree-only pre-decision of the directed entry (u, v) with degrees du, dv
 // (GPUSCAN++/GPUScan.cu:143-169; orientation GPUSCAN++/Graph.cpp:75-83): the

@@ -73,7 +73,8 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                  const uint32_t* __restrict__ src,
-                                                 double eps, bool count_all, uint8_t* __restrict__ flag,
+                                                 double eps, bool count_all, PairFilter pf,
+                                                 uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
                                                  uint64_t* __restrict__ cnt) {
   const uint32_t lane = lane_id();
@@ -93,6 +94,11 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       if (count_all && f <= kFlagY) {  // every edge to its owner, for exact counts
         const uint32_t du = re[u] - rb[u];
         f = (dv > du || (dv == du && v > u)) ? kFlagUUp : kFlagUDown;
+      }
+      if (pf.on && f == kFlagUDown) {
+        const bool u0 = u - pf.lo0 < pf.hi0 - pf.lo0, u1 = u - pf.lo1 < pf.hi1 - pf.lo1;
+        const bool v0 = v - pf.lo0 < pf.hi0 - pf.lo0, v1 = v - pf.lo1 < pf.hi1 - pf.lo1;
+        if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
       }
       flag[k] = f;
       if (f == kFlagUDown) {
@@ -662,6 +668,7 @@ __global__ void __launch_bounds__(T::kThreads)
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
   k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, count_all,
+                                                   g->pair,
                                                    b.flag, b.cnt_u, b.work_u, b.cnt);
 }
 

@@ -238,21 +238,22 @@ typedef struct {
 /* Replaces the out-of-core GPUScan of GPUSCAN++O (SCANoffload.cu:922-1464,
  * chips GPUSCAN++O/Graph.cpp:414-576) for a host-resident graph of any size
  * (m may exceed 2^32; n < 2^31): at most `device_budget` bytes of device
- * memory are allocated; the similarity phase runs over chips of owner rows
- * plus the short rows their undecided edges read, the later phases over chips
- * of rows, with the per-vertex state resident throughout.  Outputs as
- * sgc_scan (host arrays, each optional) and bit-identical to it.
- * stats->ms_* similarity fields are device time summed over chips; the
- * per-phase fields are filled when all rows fit one chip; ms_total is wall
- * time.  SGC_ENOMEM if the budget cannot hold the per-vertex state and the
- * largest single row's chip. */
+ * memory are allocated.  Rows are cut into blocks of at most half a chip; the
+ * similarity phase runs over every pair of blocks (each undecided edge in the
+ * chip holding both of its rows), the later phases over chips of rows, with
+ * the per-vertex state resident throughout.  Outputs as sgc_scan (host arrays,
+ * each optional) and bit-identical to it.  stats->ms_similarity fields are
+ * device time summed over chips; the per-phase fields are filled when all
+ * rows fit one chip; ms_total is wall time.  SGC_ENOMEM if the budget is
+ * below sgc_ooc_min_budget. */
 int sgc_scan_ooc(sgc_ctx* ctx, const sgc_host_graph* g, double eps, int32_t mu, uint64_t device_budget,
                  int32_t* labels, uint8_t* core, uint8_t* sim, sgc_stats* stats, sgc_ooc_stats* ooc);
 
-/* Smallest device budget sgc_scan_ooc accepts for g at eps: the per-vertex
- * state plus the chip of the single most demanding owner row (its row and the
- * short rows of its undecided edges).  No counterpart in the reference, whose
- * chip planner takes a fixed budget and fails inside. */
+/* Smallest device budget sgc_scan_ooc accepts for g: the per-vertex state
+ * plus a chip of two row blocks that each hold the longest row (eps is
+ * accepted for future plans and does not change the result today).  No
+ * counterpart in the reference, whose chip planner takes a fixed budget and
+ * fails inside. */
 int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t* bytes);
 
 /* ------------------------------------------------------------------------ */
