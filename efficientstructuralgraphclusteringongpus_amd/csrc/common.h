@@ -22,6 +22,9 @@ inline double now_ms() {
   return duration_cast<duration<double, std::milli>>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Stage progress of long host/device runs on stderr when SGC_PROGRESS is set.
+void progress(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
 // Plain or pinned host allocation; pinned memory comes from hipHostMalloc so
 // that partitions can be streamed to the device with asynchronous copies.
 void* host_alloc(size_t bytes, bool pinned);

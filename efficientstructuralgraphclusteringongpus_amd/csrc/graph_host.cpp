@@ -32,6 +32,18 @@ int fail(int code, const char* fmt, ...) {
 
 void clear_error() { g_last_error.clear(); }
 
+void progress(const char* fmt, ...) {
+  static const bool on = getenv("SGC_PROGRESS") != nullptr;
+  if (!on) return;
+  va_list ap;
+  va_start(ap, fmt);
+  fprintf(stderr, "[sgc %.1f s] ", now_ms() / 1e3);
+  vfprintf(stderr, fmt, ap);
+  fputc('\n', stderr);
+  fflush(stderr);
+  va_end(ap);
+}
+
 void* host_alloc(size_t bytes, bool pinned) {
   if (bytes == 0) bytes = 8;
   if (pinned) {
@@ -150,6 +162,7 @@ int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint
     __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
   }
   if (bad) return fail(SGC_EINVAL, "edge endpoint >= n");
+  progress("graph build: degrees counted (%llu pairs)", (unsigned long long)count);
   std::vector<uint64_t> off(n + 1);
   off[0] = 0;
   for (uint64_t u = 0; u < n; ++u) off[u + 1] = off[u] + cnt[u];
@@ -166,6 +179,7 @@ int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint
     buf[__atomic_fetch_add(&fill[u], 1, __ATOMIC_RELAXED)] = v;
     buf[__atomic_fetch_add(&fill[v], 1, __ATOMIC_RELAXED)] = u;
   }
+  progress("graph build: entries scattered (%llu)", (unsigned long long)total);
   std::vector<uint64_t> deg(n + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (uint64_t u = 0; u < n; ++u) {
@@ -176,6 +190,7 @@ int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint
   }
   uint64_t m = 0;
   for (uint64_t u = 0; u < n; ++u) m += deg[u];
+  progress("graph build: rows sorted, m = %llu", (unsigned long long)m);
   sgc_host_graph* g = new_host_graph(n, m, false);
   if (!g) {
     free(buf);
@@ -419,6 +434,7 @@ extern "C" int sgc_graph_rmat(int scale, double edge_factor, double a, double b,
   }
 #pragma omp parallel for schedule(static, 65536)
   for (uint64_t i = 0; i < draws; ++i) rmat_draw(i, scale, seed, t, &src[i], &dst[i]);
+  progress("rmat scale %d: %llu draws", scale, (unsigned long long)draws);
   int rc = build_from_pairs(n, draws, src, dst, out);
   free(src);
   free(dst);

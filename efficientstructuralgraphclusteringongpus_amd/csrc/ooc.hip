@@ -360,10 +360,12 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       }
       prev = c;
       ++chip_no;
+      if (chip_no % 32 == 0) progress("ooc: %zu of %zu similarity chips issued", chip_no, P * (P + 1) / 2);
     }
   }
   if (chip_no && (rc = merge(prev, chip_no - 1))) return rc;
   const size_t nsim = chip_no;
+  progress("ooc: similarity done over %zu chips of %zu blocks", nsim, P);
 
   // ---- core check, union-find, border, classify over row chips
   const double t_phase = now_ms();
@@ -414,12 +416,14 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     launch_core_count(&v, rb, s);
   }
   launch_core_mark(&whole, rb, p, s);
+  progress("ooc: core check issued over %zu row chips", nrow);
   SGC_HIP(hipEventRecord(ctx->ev[4], s));
   for (size_t i = 0; i < nrow; ++i) {
     if ((rc = load_rows(nrow - 1 - i, &v))) return rc;  // last chip first: it is still loaded
     launch_union(&v, rb, s);
   }
   launch_flatten(&whole, rb, s);
+  progress("ooc: union-find issued");
   SGC_HIP(hipEventRecord(ctx->ev[5], s));
   for (size_t i = 0; i < nrow; ++i) {
     if ((rc = load_rows(i, &v))) return rc;
