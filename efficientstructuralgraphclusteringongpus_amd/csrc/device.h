@@ -100,6 +100,11 @@ struct sgc_dgraph {
   const uint32_t* row_end = nullptr;
   sgc::SimTuning tuning;
   sgc::PairFilter pair;
+  // Window-offset index of a resident graph with n > 2^20 (null otherwise):
+  // (wcount + 1) positions per row, where the row crosses each 2^20-id
+  // window boundary; the workgroup tier then streams exact window slices.
+  uint32_t* woff = nullptr;
+  uint32_t wcount = 0;
 };
 
 namespace sgc {
@@ -141,6 +146,10 @@ size_t scan_tmp_words(uint64_t nitems);
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
+// 2^20-id windows of the large-tier bitmap covering n ids, and the kernel
+// filling g->woff (see sgc_dgraph).
+uint32_t window_count(uint32_t n);
+void launch_window_offsets(const sgc_dgraph* g, hipStream_t s);
 void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 // count_all: every edge is left to its owner (for exact common-neighbour

@@ -163,6 +163,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   }
   d->max_degree = (uint32_t)maxd;
   auto cleanup = [d]() {
+    if (d->woff) (void)hipFree(d->woff);
     if (d->row_ptr) (void)hipFree(d->row_ptr);
     if (d->adj) (void)hipFree(d->adj);
     if (d->src) (void)hipFree(d->src);
@@ -178,11 +179,24 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   }
   d->row_begin = d->row_ptr;
   d->row_end = d->row_ptr + 1;
+  // window-offset index when there is more than one window and it is no
+  // larger than the adjacency itself
+  const uint32_t W = window_count(d->n);
+  const uint64_t woff_bytes = (uint64_t)d->n * (W + 1) * 4;
+  if (W > 1 && woff_bytes <= std::max<uint64_t>(m_bytes, 1ull << 30)) {
+    if (hipMalloc(&d->woff, woff_bytes) == hipSuccess) {
+      d->wcount = W;
+    } else {
+      (void)hipGetLastError();
+      d->woff = nullptr;  // the kernels find window boundaries while streaming instead
+    }
+  }
   hipError_t e = hipMemcpyAsync(d->row_ptr, rp32.data(), (g->n + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess && g->m)
     e = hipMemcpyAsync(d->adj, g->adj, g->m * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
     launch_fill_src(d, ctx->stream);
+    launch_window_offsets(d, ctx->stream);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -198,6 +212,7 @@ extern "C" void sgc_dgraph_free(sgc_dgraph* d) {
   if (!d) return;
   (void)hipSetDevice(d->ctx->device);
   (void)hipStreamSynchronize(d->ctx->stream);
+  if (d->woff) (void)hipFree(d->woff);
   if (d->row_ptr) (void)hipFree(d->row_ptr);
   if (d->adj) (void)hipFree(d->adj);
   if (d->src) (void)hipFree(d->src);

@@ -470,14 +470,14 @@ __global__ void __launch_bounds__(T::kThreads)
     k_sim_units(uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
                 const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
-                uint32_t* __restrict__ cn_out,
+                uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
                 const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
                 int units_slot, int queue_slot) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
   __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
   __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
-  __shared__ uint32_t e_cap[T::kMaxEdges];
+  __shared__ uint32_t e_cap[T::kMaxEdges], e_a[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend;
   const uint32_t lane = lane_id();
   const uint64_t nunits = cnt[units_slot];
@@ -499,15 +499,16 @@ __global__ void __launch_bounds__(T::kThreads)
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
       const uint32_t k = ulist[un.x + t];
-      uint32_t ab = 0, s = 0, cstar = 0;
+      uint32_t ab = 0, s = 0, cstar = 0, a = 0;
       if (k < m) {
-        const uint32_t a = (uint32_t)adj[k];
+        a = (uint32_t)adj[k];
         ab = rb[a];
         s = re[a] - ab;
         const double rhs = predicate_rhs(eps, s, l);
         cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
       }
       e_k[t] = k;
+      e_a[t] = a;
       e_ab[t] = ab;
       e_s[t] = s;
       e_cur[t] = ab;
@@ -546,15 +547,28 @@ __global__ void __launch_bounds__(T::kThreads)
         for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
       }
       const bool last = kend >= be;
+      const bool exact = T::kBitmap && woff;  // slices known from the window-offset index
       const bool mir_here = b - wlo < (1u << kWindowBits);  // large tier: the window holding id b
-      const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last, mir_here};
+      const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
         const uint32_t cur = e_cur[t], ab = e_ab[t], s = e_s[t];
         e_pos[t] = cur;
         e_hi[t] = ab + s;
         uint32_t cap = T::kChunk;
-        if (!last) {
+        if (exact) {
+          // the edge's slice of this window; the elements skipped since the
+          // last window lie in windows without owner keys: consumed, no hits
+          uint32_t lo = cur, hi = cur;
+          if (e_k[t] < m) {
+            const uint32_t* wo = woff + (size_t)e_a[t] * wstride + (wlo >> kWindowBits);
+            lo = wo[0];
+            hi = wo[1];
+            if (lo > cur) e_state[t] += (unsigned long long)(lo - cur);
+          }
+          e_pos[t] = lo;
+          e_hi[t] = hi;
+        } else if (!last) {
           // the short row's slice is taken to end where the owner row's
           // share of values below whi puts it
           const uint32_t pe = ab + (uint32_t)((uint64_t)s * (kend - bb) / l);
@@ -663,7 +677,44 @@ __global__ void __launch_bounds__(T::kThreads)
   }
 }
 
+// Window-offset index: woff[x * (W + 1) + w] = first position of row x whose
+// value is >= w * 2^20 (w = 0: the row start, w = W: its end).  One thread
+// per (row, window) boundary.
+__global__ void __launch_bounds__(kBlock) k_window_offsets(uint32_t n, const uint32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ adj, uint32_t W,
+                                                           uint32_t* __restrict__ woff) {
+  const uint64_t total = (uint64_t)n * (W + 1);
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += stride) {
+    const uint32_t x = (uint32_t)(i / (W + 1)), w = (uint32_t)(i % (W + 1));
+    uint32_t lo = rp[x], hi = rp[x + 1];
+    if (w == 0) {
+      hi = lo;
+    } else if (w < W) {
+      const int32_t key = (int32_t)(w << kWindowBits);
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (adj[mid] < key)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+    } else {
+      lo = hi;
+    }
+    woff[i] = lo;
+  }
+}
+
 }  // namespace
+
+uint32_t window_count(uint32_t n) { return (uint32_t)(((uint64_t)n + (1u << kWindowBits) - 1) >> kWindowBits); }
+
+void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
+  if (!g->woff || g->n == 0) return;
+  const uint64_t total = (uint64_t)g->n * (g->wcount + 1);
+  k_window_offsets<<<grid_for(total, kBlock), kBlock, 0, s>>>(g->n, g->row_ptr, g->adj, g->wcount, g->woff);
+}
 
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
@@ -702,12 +753,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
-                                                             b.flag, cn_out, b.ulist, b.units2, b.cnt, kCntUnits2,
-                                                             kCntQueue2);
+                                                             b.flag, cn_out, g->woff, g->wcount + 1, b.ulist,
+                                                             b.units2, b.cnt, kCntUnits2, kCntQueue2);
   if (mid) (void)hipEventRecord(mid, s);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src,
-                                                                 p.eps, b.flag, cn_out, b.ulist, b.units1, b.cnt,
-                                                                 kCntUnits1, kCntQueue1);
+                                                                 p.eps, b.flag, cn_out, nullptr, 0, b.ulist, b.units1,
+                                                                 b.cnt, kCntUnits1, kCntQueue1);
 }
 
 }  // namespace sgc
