@@ -35,7 +35,8 @@ _LIB = None
 
 
 def library_path() -> str:
-    return os.path.join(_PKG, "lib", "libsgc.so")
+    """lib/libsgc.so of this package, or the build named by SGC_LIBRARY."""
+    return os.environ.get("SGC_LIBRARY") or os.path.join(_PKG, "lib", "libsgc.so")
 
 
 class SGCError(RuntimeError):

@@ -38,6 +38,7 @@ enum Counter {
   kCntQueueE,   // next edge for the one-wave-per-edge kernel
   kCntLoaded,   // short-row elements loaded by the claims (consumed or not)
   kCntClaims,   // claims taken
+  kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32
 };
 

@@ -292,6 +292,14 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
     st->probe_elements = (double)c[kCntProbes];
     st->loaded_elements = (double)c[kCntLoaded];
     st->claims = (double)c[kCntClaims];
+#ifdef SGC_PHASE_TIMERS
+    fprintf(stderr, "phase ticks (unit, window, stream, barrier, final) large: %llu %llu %llu %llu %llu small: "
+            "%llu %llu %llu %llu %llu\n", (unsigned long long)c[kCntTimer0], (unsigned long long)c[kCntTimer0 + 1],
+            (unsigned long long)c[kCntTimer0 + 2], (unsigned long long)c[kCntTimer0 + 3],
+            (unsigned long long)c[kCntTimer0 + 4], (unsigned long long)c[kCntTimer0 + 5],
+            (unsigned long long)c[kCntTimer0 + 6], (unsigned long long)c[kCntTimer0 + 7],
+            (unsigned long long)c[kCntTimer0 + 8], (unsigned long long)c[kCntTimer0 + 9]);
+#endif
   }
   return SGC_OK;
 }

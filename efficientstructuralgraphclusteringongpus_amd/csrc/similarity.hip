@@ -388,13 +388,38 @@ struct ClaimTable {
   bool mir_here;         // the window holds id b
 };
 
+// Workgroup-tier owners of degree <= kCuckooMaxKeys hold their whole row in
+// LDS as a two-table cuckoo hash (2 x 16384 slots, at most a quarter full):
+// every lookup is exactly two LDS reads and there are no windows, so claims
+// are not cut at window boundaries.  Insertion evicts along a chain of at
+// most kCuckooKicks; a failed chain (never observed at this load) makes the
+// unit fall back to bitmap windows.
+constexpr uint32_t kCuckooSlots = 16384;
+constexpr uint32_t kCuckooMaxKeys = 8192;
+constexpr int kCuckooKicks = 128;
+
+__device__ __forceinline__ uint32_t cuckoo_h0(uint32_t x) { return (x * 0x9E3779B1u) >> 18; }
+__device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x) { return ((x ^ (x >> 16)) * 0x85EBCA77u) >> 18; }
+
+__device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key) {
+  uint32_t k = key, side = 0;
+  for (int i = 0; i < kCuckooKicks; ++i) {
+    const uint32_t slot = side ? kCuckooSlots + cuckoo_h1(k) : cuckoo_h0(k);
+    const uint32_t old = atomicExch(&tab[slot], k);
+    if (old == kEmptySlot) return true;
+    k = old;  // the evicted key moves to its slot in the other table
+    side ^= 1u;
+  }
+  return false;
+}
+
 // One claim of NL = ceil(len / 64) wave loads, straight-line so that all its
 // global loads, then all its LDS reads, are in flight together.  Buffer loads
 // past the claim's end read 0 without touching memory; those lanes are kept
 // out of every count.
 // Returns the hits (uniform); *used = elements consumed (large tier: up to the
 // window's end, which is also lowered into *ehi when the claim crosses it).
-template <class T, uint32_t NL>
+template <class T, uint32_t NL, bool CUCKOO>
 __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0, uint32_t len, uint32_t* mir,
                                                 uint32_t* ehi, uint32_t* used_out) {
   const uint32_t lane = lane_id();
@@ -403,7 +428,21 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
 #pragma unroll
   for (uint32_t j = 0; j < NL; ++j) x[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + j * kWave) * 4u, 0, 0);
   uint32_t h = 0, used = len;
-  if constexpr (T::kBitmap) {
+  if constexpr (CUCKOO) {
+    record_mirror<NL>(x, len, c.b, s0, mir);
+    uint32_t v0[NL], v1[NL];
+#pragma unroll
+    for (uint32_t j = 0; j < NL; ++j) {
+      v0[j] = c.tab[cuckoo_h0((uint32_t)x[j])];
+      v1[j] = c.tab[kCuckooSlots + cuckoo_h1((uint32_t)x[j])];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < NL; ++j) {
+      uint64_t hm = __ballot(v0[j] == (uint32_t)x[j] || v1[j] == (uint32_t)x[j]);
+      if (j + 1 == NL) hm &= load_mask(j, len);  // lanes past the claim read id 0
+      h += (uint32_t)__popcll(hm);
+    }
+  } else if constexpr (T::kBitmap) {
     // first position whose value reaches whi (sorted row): from there on the
     // elements belong to later windows and probe the zero word
     if (!c.last) {
@@ -437,15 +476,15 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
 }
 
 // count_claim<T, nl> for a runtime nl in [LO, HI], by binary dispatch.
-template <class T, uint32_t LO = 1, uint32_t HI = T::kLoads>
+template <class T, bool CUCKOO, uint32_t LO = 1, uint32_t HI = T::kLoads>
 __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable& c, uint32_t s0, uint32_t len,
                                                    uint32_t* mir, uint32_t* ehi, uint32_t* used) {
   if constexpr (LO == HI) {
-    return count_claim<T, LO>(c, s0, len, mir, ehi, used);
+    return count_claim<T, LO, CUCKOO>(c, s0, len, mir, ehi, used);
   } else {
     constexpr uint32_t MID = (LO + HI) / 2;
-    if (nl <= MID) return count_dispatch<T, LO, MID>(nl, c, s0, len, mir, ehi, used);
-    return count_dispatch<T, MID + 1, HI>(nl, c, s0, len, mir, ehi, used);
+    if (nl <= MID) return count_dispatch<T, CUCKOO, LO, MID>(nl, c, s0, len, mir, ehi, used);
+    return count_dispatch<T, CUCKOO, MID + 1, HI>(nl, c, s0, len, mir, ehi, used);
   }
 }
 
@@ -467,7 +506,7 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
 __global__ void __launch_bounds__(T::kThreads)
-    k_sim_units(uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+    k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
                 const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
@@ -478,10 +517,29 @@ __global__ void __launch_bounds__(T::kThreads)
   __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
   __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
   __shared__ uint32_t e_cap[T::kMaxEdges], e_a[T::kMaxEdges];
-  __shared__ uint32_t s_unit, s_cursor, s_kend;
+  __shared__ uint32_t s_unit, s_cursor, s_kend, s_fail;
+  // the owner row's window boundaries, from the window-offset index
+  constexpr uint32_t kMaxOwnerWindows = 64;
+  __shared__ uint32_t s_bw[T::kBitmap ? kMaxOwnerWindows + 1 : 1];
+  const bool bw = T::kBitmap && woff && wstride <= kMaxOwnerWindows + 1;
   const uint32_t lane = lane_id();
   const uint64_t nunits = cnt[units_slot];
   uint64_t streamed = 0, loaded = 0, claims = 0;
+#ifdef SGC_PHASE_TIMERS
+  // debug builds: wall-clock ticks per wave in unit set-up, window set-up,
+  // streaming, the end-of-window barrier and the final writes
+  uint64_t tm[5] = {0, 0, 0, 0, 0}, t_ = wall_clock64();
+#define SGC_TICK(i)                   \
+  do {                                \
+    const uint64_t now_ = wall_clock64(); \
+    tm[i] += now_ - t_;               \
+    t_ = now_;                        \
+  } while (0)
+#else
+#define SGC_TICK(i) \
+  do {              \
+  } while (0)
+#endif
   for (;;) {
     if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
     __syncthreads();
@@ -516,24 +574,42 @@ __global__ void __launch_bounds__(T::kThreads)
       e_mir[t] = kNone;
       e_state[t] = 0ull;
     }
+    if (bw && threadIdx.x < wstride) s_bw[threadIdx.x] = woff[(size_t)b * wstride + threadIdx.x];
+    SGC_TICK(0);
+    // one window covers the ids: its exact bitmap is cheaper than the hash
+    bool cuckoo = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
     uint32_t kcur = bb;
     while (kcur < be) {
       // ---- LDS table for owner keys adj[kcur, kend).  Large tier: exact
       // bitmap of the 2^20-id window [wlo, whi) holding adj[kcur]; values >=
       // whi belong to later windows.  Small tier: hash table of the whole row.
       uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0;
-      if constexpr (T::kBitmap) {
+      if (T::kBitmap && cuckoo) {
+        uint4* tab4 = reinterpret_cast<uint4*>(tab);
+        for (uint32_t t = threadIdx.x; t < 2 * kCuckooSlots / 4; t += T::kThreads)
+          tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
+        if (threadIdx.x == 0) s_fail = 0;
+        __syncthreads();
+        bool ok = true;
+        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) ok = cuckoo_insert(tab, (uint32_t)adj[t]) && ok;
+        if (!ok) s_fail = 1;
+        __syncthreads();
+        if (s_fail) {  // retry this unit with bitmap windows
+          cuckoo = false;
+          continue;
+        }
+      } else if constexpr (T::kBitmap) {
         const uint32_t w = (uint32_t)adj[kcur] >> kWindowBits;
         wlo = w << kWindowBits;
         whi = wlo + (1u << kWindowBits);
-        if (threadIdx.x < kWave) {
+        if (threadIdx.x < kWave && !bw) {
           const uint32_t e = whi > 0x7FFFFFFFu ? be : wave_lower_bound(adj, kcur, be, (int32_t)whi);
           if (lane == 0) s_kend = e;
         }
         uint4* tab4 = reinterpret_cast<uint4*>(tab);
         for (uint32_t t = threadIdx.x; t < T::kTableWords / 4; t += T::kThreads) tab4[t] = make_uint4(0, 0, 0, 0);
         __syncthreads();
-        kend = s_kend;
+        kend = bw ? s_bw[w + 1] : s_kend;
         for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) {
           const uint32_t x = (uint32_t)adj[t] - wlo;
           atomicOr(&tab[x >> 5], 1u << (x & 31));
@@ -547,8 +623,8 @@ __global__ void __launch_bounds__(T::kThreads)
         for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
       }
       const bool last = kend >= be;
-      const bool exact = T::kBitmap && woff;  // slices known from the window-offset index
-      const bool mir_here = b - wlo < (1u << kWindowBits);  // large tier: the window holding id b
+      const bool exact = T::kBitmap && woff && !cuckoo;  // slices known from the window-offset index
+      const bool mir_here = cuckoo || b - wlo < (1u << kWindowBits);  // the table holds id b's range
       const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
@@ -580,6 +656,7 @@ __global__ void __launch_bounds__(T::kThreads)
       }
       if (threadIdx.x == 0) s_cursor = 0;
       __syncthreads();  // table and per-edge cursors ready
+      SGC_TICK(1);
       // ---- stream the short rows
       uint32_t cur_e = kNone;
       for (;;) {
@@ -626,13 +703,20 @@ __global__ void __launch_bounds__(T::kThreads)
         const uint32_t s0 = __builtin_amdgcn_readfirstlane(start);
         const uint32_t len = __builtin_amdgcn_readfirstlane(end - start);
         uint32_t used;
-        const uint32_t h = count_dispatch<T>((len + kWave - 1) / kWave, cc, s0, len, &e_mir[e], &e_hi[e], &used);
+        const uint32_t nl = (len + kWave - 1) / kWave;
+        uint32_t h;
+        if (T::kBitmap && cuckoo)
+          h = count_dispatch<T, T::kBitmap>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
+        else
+          h = count_dispatch<T, false>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
         if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
         streamed += used;
         loaded += len;
         ++claims;
       }
+      SGC_TICK(2);
       __syncthreads();
+      SGC_TICK(3);
       if (threadIdx.x < ne) e_cur[threadIdx.x] = e_hi[threadIdx.x];
       kcur = kend;
       __syncthreads();
@@ -669,7 +753,14 @@ __global__ void __launch_bounds__(T::kThreads)
       }
     }
     __syncthreads();
+    SGC_TICK(4);
   }
+#ifdef SGC_PHASE_TIMERS
+  if (lane == 0)
+    for (int i = 0; i < 5; ++i)
+      atomicAdd((unsigned long long*)&cnt[kCntTimer0 + (T::kBitmap ? 0 : 5) + i], (unsigned long long)tm[i]);
+#endif
+#undef SGC_TICK
   if (lane == 0 && claims) {
     atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
     atomicAdd((unsigned long long*)&cnt[kCntLoaded], (unsigned long long)loaded);
@@ -752,11 +843,11 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
-  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
                                                              b.flag, cn_out, g->woff, g->wcount + 1, b.ulist,
                                                              b.units2, b.cnt, kCntUnits2, kCntQueue2);
   if (mid) (void)hipEventRecord(mid, s);
-  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src,
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->src,
                                                                  p.eps, b.flag, cn_out, nullptr, 0, b.ulist, b.units1,
                                                                  b.cnt, kCntUnits1, kCntQueue1);
 }
