@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["graph_host.cpp", "ooc_host.cpp", "result_io.cpp"]
-HIP_SOURCES = ["kernels.hip", "similarity.hip", "device_api.hip", "ooc.hip", "serve.hip", "batch.hip"]
+HIP_SOURCES = ["kernels.hip", "similarity.hip", "device_api.hip", "ooc.hip", "serve.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, "-Wall", "-Wno-unused-function"]
 HOST_FLAGS = COMMON + ["-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
@@ -68,7 +68,7 @@ def build(verbose: bool = True) -> str:
     """Compiles every source and links lib/libsgc.so and lib/GPUScan."""
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
-    sources = [s for s in HOST_SOURCES + HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    sources = HOST_SOURCES + HIP_SOURCES
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))
     lib = os.path.join(LIBDIR, "libsgc.so")
