@@ -385,15 +385,16 @@ struct ClaimTable {
   uint32_t mask, shift;  // small tier: hash table shape
   uint32_t b;            // the owner
   bool last;             // no later window
-  bool mir_here;         // the window holds id b
+  bool mir_here;         // the table's id range holds b
 };
 
 // Workgroup-tier owners of degree <= kCuckooMaxKeys hold their whole row in
-// LDS as a two-table cuckoo hash (2 x 16384 slots, at most a quarter full):
-// every lookup is exactly two LDS reads and there are no windows, so claims
-// are not cut at window boundaries.  Insertion evicts along a chain of at
-// most kCuckooKicks; a failed chain (never observed at this load) makes the
-// unit fall back to bitmap windows.
+// LDS as a two-table cuckoo hash (2 x 16384 slots, at most half full), and
+// longer rows hold runs of consecutive windows with at most that many keys
+// the same way: every lookup is exactly two LDS reads, and a claim is cut at
+// the run's end instead of at every window's.  Insertion evicts along a
+// chain of at most kCuckooKicks; a failed chain (never observed at this
+// load) makes the unit fall back to bitmap windows.
 constexpr uint32_t kCuckooSlots = 16384;
 constexpr uint32_t kCuckooMaxKeys = 8192;
 constexpr int kCuckooKicks = 128;
@@ -429,7 +430,7 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
   for (uint32_t j = 0; j < NL; ++j) x[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + j * kWave) * 4u, 0, 0);
   uint32_t h = 0, used = len;
   if constexpr (CUCKOO) {
-    record_mirror<NL>(x, len, c.b, s0, mir);
+    if (c.mir_here) record_mirror<NL>(x, len, c.b, s0, mir);
     uint32_t v0[NL], v1[NL];
 #pragma unroll
     for (uint32_t j = 0; j < NL; ++j) {
@@ -489,8 +490,9 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 }
 
 // One unit = one owner b and up to MAX_EDGES of its undecided edges (a, b).
-// The owner row N(b) goes into LDS (small tier: hash table; large tier: one
-// 2^20-id window at a time as an exact bitmap).  The short rows N(a) are
+// The owner row N(b) goes into LDS (small tier: hash table; large tier: the
+// whole row or runs of 2^20-id windows as a cuckoo hash, else one window at a
+// time as an exact bitmap).  The short rows N(a) are
 // streamed in claims that every wave of the workgroup takes dynamically: a
 // wave walks one edge in order, claiming each time just enough elements for a
 // decision to become possible, takes the next unstarted edge when its edge
@@ -575,44 +577,65 @@ __global__ void __launch_bounds__(T::kThreads)
       e_state[t] = 0ull;
     }
     if (bw && threadIdx.x < wstride) s_bw[threadIdx.x] = woff[(size_t)b * wstride + threadIdx.x];
+    if constexpr (T::kBitmap) __syncthreads();  // every thread picks the passes from s_bw
     SGC_TICK(0);
-    // one window covers the ids: its exact bitmap is cheaper than the hash
-    bool cuckoo = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
+    // Large tier, per pass over the owner row: the whole row as a cuckoo
+    // hash when it has at most kCuckooMaxKeys keys and the ids span more than
+    // one window; else, with the window-offset index, a cuckoo hash of the
+    // longest run of two or more consecutive windows holding at most
+    // kCuckooMaxKeys keys; else the exact bitmap of one window.  A failed
+    // cuckoo build sends the rest of the unit to bitmap windows.
+    bool hash_ok = T::kBitmap;
+    const bool whole = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
+    const uint32_t nwin = bw ? wstride - 1 : 0;
     uint32_t kcur = bb;
     while (kcur < be) {
-      // ---- LDS table for owner keys adj[kcur, kend).  Large tier: exact
-      // bitmap of the 2^20-id window [wlo, whi) holding adj[kcur]; values >=
-      // whi belong to later windows.  Small tier: hash table of the whole row.
-      uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0;
-      if (T::kBitmap && cuckoo) {
-        uint4* tab4 = reinterpret_cast<uint4*>(tab);
-        for (uint32_t t = threadIdx.x; t < 2 * kCuckooSlots / 4; t += T::kThreads)
-          tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
-        if (threadIdx.x == 0) s_fail = 0;
-        __syncthreads();
-        bool ok = true;
-        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) ok = cuckoo_insert(tab, (uint32_t)adj[t]) && ok;
-        if (!ok) s_fail = 1;
-        __syncthreads();
-        if (s_fail) {  // retry this unit with bitmap windows
-          cuckoo = false;
-          continue;
+      // ---- LDS table for owner keys adj[kcur, kend) of the ids [wlo, whi)
+      // (windows w0 .. w0 + wspan - 1).  Small tier: hash table of the whole row.
+      uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0, wspan = 1;
+      bool hashed = false;
+      if constexpr (T::kBitmap) {
+        uint32_t w0 = 0, w1 = 0;
+        if (hash_ok && whole) {
+          hashed = true;
+        } else {
+          w0 = (uint32_t)adj[kcur] >> kWindowBits;
+          w1 = w0 + 1;
+          if (hash_ok && nwin > 1)
+            while (w1 < nwin && s_bw[w1 + 1] - kcur <= kCuckooMaxKeys) ++w1;
+          hashed = w1 >= w0 + 2;
+          wlo = w0 << kWindowBits;
+          whi = w1 < nwin || !bw ? wlo + ((w1 - w0) << kWindowBits) : 0xFFFFFFFFu;
+          wspan = w1 - w0;
         }
-      } else if constexpr (T::kBitmap) {
-        const uint32_t w = (uint32_t)adj[kcur] >> kWindowBits;
-        wlo = w << kWindowBits;
-        whi = wlo + (1u << kWindowBits);
-        if (threadIdx.x < kWave && !bw) {
-          const uint32_t e = whi > 0x7FFFFFFFu ? be : wave_lower_bound(adj, kcur, be, (int32_t)whi);
-          if (lane == 0) s_kend = e;
-        }
-        uint4* tab4 = reinterpret_cast<uint4*>(tab);
-        for (uint32_t t = threadIdx.x; t < T::kTableWords / 4; t += T::kThreads) tab4[t] = make_uint4(0, 0, 0, 0);
-        __syncthreads();
-        kend = bw ? s_bw[w + 1] : s_kend;
-        for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) {
-          const uint32_t x = (uint32_t)adj[t] - wlo;
-          atomicOr(&tab[x >> 5], 1u << (x & 31));
+        if (hashed) {
+          if (!whole) kend = s_bw[w1];
+          uint4* tab4 = reinterpret_cast<uint4*>(tab);
+          for (uint32_t t = threadIdx.x; t < 2 * kCuckooSlots / 4; t += T::kThreads)
+            tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
+          if (threadIdx.x == 0) s_fail = 0;
+          __syncthreads();
+          bool ok = true;
+          for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) ok = cuckoo_insert(tab, (uint32_t)adj[t]) && ok;
+          if (!ok) s_fail = 1;
+          __syncthreads();
+          if (s_fail) {  // redo this pass with bitmap windows
+            hash_ok = false;
+            continue;
+          }
+        } else {
+          if (threadIdx.x < kWave && !bw) {
+            const uint32_t e = whi > 0x7FFFFFFFu ? be : wave_lower_bound(adj, kcur, be, (int32_t)whi);
+            if (lane == 0) s_kend = e;
+          }
+          uint4* tab4 = reinterpret_cast<uint4*>(tab);
+          for (uint32_t t = threadIdx.x; t < T::kTableWords / 4; t += T::kThreads) tab4[t] = make_uint4(0, 0, 0, 0);
+          __syncthreads();
+          kend = bw ? s_bw[w0 + 1] : s_kend;
+          for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) {
+            const uint32_t x = (uint32_t)adj[t] - wlo;
+            atomicOr(&tab[x >> 5], 1u << (x & 31));
+          }
         }
       } else {
         const uint32_t TS = table_size_for(2 * l);
@@ -623,8 +646,9 @@ __global__ void __launch_bounds__(T::kThreads)
         for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
       }
       const bool last = kend >= be;
-      const bool exact = T::kBitmap && woff && !cuckoo;  // slices known from the window-offset index
-      const bool mir_here = cuckoo || b - wlo < (1u << kWindowBits);  // the table holds id b's range
+      // slices known from the window-offset index (every pass but a whole-row hash)
+      const bool exact = T::kBitmap && woff && !(hashed && whole);
+      const bool mir_here = (hashed && whole) || b - wlo < whi - wlo;  // the table holds id b's range
       const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
@@ -639,7 +663,7 @@ __global__ void __launch_bounds__(T::kThreads)
           if (e_k[t] < m) {
             const uint32_t* wo = woff + (size_t)e_a[t] * wstride + (wlo >> kWindowBits);
             lo = wo[0];
-            hi = wo[1];
+            hi = wo[wspan];
             if (lo > cur) e_state[t] += (unsigned long long)(lo - cur);
           }
           e_pos[t] = lo;
@@ -705,7 +729,7 @@ __global__ void __launch_bounds__(T::kThreads)
         uint32_t used;
         const uint32_t nl = (len + kWave - 1) / kWave;
         uint32_t h;
-        if (T::kBitmap && cuckoo)
+        if (T::kBitmap && hashed)
           h = count_dispatch<T, T::kBitmap>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
         else
           h = count_dispatch<T, false>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);

@@ -170,6 +170,54 @@ def test_bitmap_windows_for_very_long_owner_rows(ctx):
     assert np.array_equal(r.sim, sim) and np.array_equal(r.labels, labels)
 
 
+def test_window_runs_in_one_hash(ctx):
+    """n > 3 * 2^20 (four windows): owner rows over 8192 keys are cut into runs
+    of consecutive windows holding at most 8192 keys (one cuckoo hash per
+    run) and single windows beyond that (bitmap); the owners, and so the
+    mirror entries, sit in different windows."""
+    say("window runs")
+    n, W = 3_300_000, 1 << 20
+    rng = np.random.default_rng(11)
+    hubs = np.array([1_500_000, 2_500_001, 3_100_000])
+
+    def spread(counts):
+        parts = [rng.choice(np.arange(w * W, min(n, (w + 1) * W)), size=c, replace=False) for w, c in enumerate(counts)]
+        return np.setdiff1d(np.concatenate(parts), hubs)
+
+    k0 = spread([2000, 3000, 3000, 4000])  # run of windows 0-2, then window 3 alone
+    k1 = np.union1d(k0[rng.random(len(k0)) < 0.8], spread([500, 500, 500, 500]))
+    k2 = spread([1000, 1000, 9000, 1000])  # run of windows 0-1, window 2 (bitmap), window 3
+    mids = np.setdiff1d(rng.choice(n, size=64, replace=False), hubs)
+    pool = np.union1d(k0, k2)
+    src, dst = [], []
+    for h, keys in zip(hubs, (k0, k1, k2)):
+        src.append(np.full(len(keys), h))
+        dst.append(keys)
+    for x in mids:
+        nb = rng.choice(pool, size=int(rng.integers(1500, 3000)), replace=False)
+        src.append(np.full(len(nb) + 3, x))
+        dst.append(np.concatenate([nb, hubs]))
+    src.append(np.array([hubs[0], hubs[1]]))
+    dst.append(np.array([hubs[1], hubs[2]]))
+    src.append(rng.integers(0, n, size=1_000_000))
+    dst.append(rng.integers(0, n, size=1_000_000))
+    g = sgc.HostGraph.from_edges(n, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32))
+    assert g.n > 3 * W and g.max_degree > 8192
+    for eps in (0.3, 0.6):
+        r = _kernels_agree(ctx, g, eps, 2, sample_size=300_000)
+    labels, core, sim, st = ob.orc_scan(np.array(g.row_ptr), np.array(g.adj), 0.6, 2)
+    assert np.array_equal(r.sim, sim) and np.array_equal(r.labels, labels)
+
+
+def test_rmat22_four_windows(ctx):
+    """R-MAT scale 22 (four 2^20-id windows): both similarity kernels agree
+    and a sample matches the oracle."""
+    say("rmat22 windows")
+    g = sgc.HostGraph.rmat(22, 16, seed=4)
+    assert g.n > 3 * (1 << 20)
+    _kernels_agree(ctx, g, 0.3, 4, sample_size=100_000)
+
+
 def test_planted_partition_hubs(ctx):
     """Hub-heavy graph: 64-vertex-scale version of the planted-partition
     workload (hub rows take the segmented workgroup tier)."""
