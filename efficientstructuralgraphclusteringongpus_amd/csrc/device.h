@@ -35,9 +35,10 @@ enum Counter {
   kCntUnits2,   // workgroup-tier work units
   kCntQueue1,   // next wave-tier unit to claim
   kCntQueue2,   // next workgroup-tier unit to claim
-  kCntQueueE,   // next edge for the one-wave-per-edge kernel
   kCntLoaded,   // short-row elements loaded by the claims (consumed or not)
   kCntClaims,   // claims taken
+  kCntDense,    // owned undecided entries listed for the one-wave-per-edge kernel
+  kCntLabeled,  // entries of labelled rows (cores, borders): picks the classify pass's side
   kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32
 };
@@ -121,10 +122,7 @@ struct PassBuffers {
   uint8_t* core;     // n
   uint32_t* cnt_u;   // n: undecided edges owned by each vertex (kFlagUDown entries)
   uint64_t* work_u;  // n: sum of the short-row lengths of those edges
-  uint32_t* fill;    // n: scatter cursors
-  uint32_t* ustart;  // n + 1: exclusive scan of cnt_u
-  uint32_t* scan_tmp;  // block sums of the scan
-  uint32_t* ulist;   // m: owned undecided entries, grouped by owner
+  uint32_t* ulist;   // m: owner b's undecided entries at [rb[b], rb[b] + cnt_u[b])
   uint2* units1;     // wave-tier units: [begin, end) ranges of ulist
   uint2* units2;     // workgroup-tier units
   uint64_t* cnt;     // kCntSlots
@@ -141,8 +139,6 @@ int carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb);
 // Units per tier are bounded by the number of owned undecided edges, which is
 // at most m; this is the capacity each unit array is allocated with.
 inline size_t unit_capacity(uint64_t m) { return (size_t)m + 1; }
-// Words of scratch the exclusive scan needs for `nitems` inputs.
-size_t scan_tmp_words(uint64_t nitems);
 
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.
@@ -176,9 +172,9 @@ void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, h
                       const int32_t* mu_dev = nullptr);
 void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
-void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+// m_all: entries of the whole graph when g is one of several row chips
+void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s, uint64_t m_all = 0);
 void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
                      const int32_t* mu_dev = nullptr);
-void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 
 }  // namespace sgc

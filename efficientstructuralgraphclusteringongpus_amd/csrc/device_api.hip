@@ -47,9 +47,6 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   const size_t o_core = take(n);
   const size_t o_cnt_u = take(n * 4);
   const size_t o_work_u = take(n * 8);
-  const size_t o_fill = take(n * 4);
-  const size_t o_ustart = take((n + 1) * 4);
-  const size_t o_scan = take(scan_tmp_words(n) * 4);
   const size_t o_ulist = take(m * 4);
   const size_t o_units1 = take(unit_capacity(m) * sizeof(uint2));
   const size_t o_units2 = take(unit_capacity(m) * sizeof(uint2));
@@ -65,9 +62,6 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   pb->core = (uint8_t*)(base + o_core);
   pb->cnt_u = (uint32_t*)(base + o_cnt_u);
   pb->work_u = (uint64_t*)(base + o_work_u);
-  pb->fill = (uint32_t*)(base + o_fill);
-  pb->ustart = (uint32_t*)(base + o_ustart);
-  pb->scan_tmp = (uint32_t*)(base + o_scan);
   pb->ulist = (uint32_t*)(base + o_ulist);
   pb->units1 = (uint2*)(base + o_units1);
   pb->units2 = (uint2*)(base + o_units2);

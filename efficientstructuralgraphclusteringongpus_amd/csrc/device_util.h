@@ -9,6 +9,9 @@
 namespace sgc {
 
 constexpr int kBlock = 256;
+// Edge passes: 64-entry chunks a wave takes per iteration, their loads all
+// issued before the gathers they feed (memory-level parallelism per wave).
+constexpr uint32_t kEdgeChunks = 4;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
 
@@ -79,6 +82,38 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     if (lane >= (uint32_t)off) v += o;
   }
   return v;
+}
+
+// Block-wide slot reservation: thread t asks for v[i] slots of counter
+// ctr[i] (i < N); on return v[i] is its first slot.  One atomic per counter
+// per call; every thread of the kBlock-thread block must call it.
+template <int N>
+__device__ __forceinline__ void block_reserve(uint32_t (&v)[N], unsigned long long* const (&ctr)[N]) {
+  __shared__ uint32_t s_part[N][kBlock / kWave];
+  __shared__ uint32_t s_base[N];
+  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+  uint32_t incl[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    incl[i] = wave_incl_scan(v[i]);
+    if (lane == kWave - 1) s_part[i][w] = incl[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < N; ++i) {
+      uint32_t t = 0;
+      for (int q = 0; q < kBlock / kWave; ++q) {
+        const uint32_t c = s_part[i][q];
+        s_part[i][q] = t;
+        t += c;
+      }
+      s_base[i] = t ? (uint32_t)atomicAdd(ctr[i], (unsigned long long)t) : 0u;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = s_base[i] + s_part[i][w] + incl[i] - v[i];
+  __syncthreads();
 }
 
 struct OpAdd {

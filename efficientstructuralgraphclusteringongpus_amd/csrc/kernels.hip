@@ -33,8 +33,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t*
 
 __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
                                                  int32_t* __restrict__ hi, uint32_t* __restrict__ cnt_u,
-                                                 uint64_t* __restrict__ work_u, uint32_t* __restrict__ fill,
-                                                 uint64_t* __restrict__ cnt) {
+                                                 uint64_t* __restrict__ work_u, uint64_t* __restrict__ cnt) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     sd[x] = 0;
@@ -42,7 +41,6 @@ __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restric
     hi[x] = -1;
     cnt_u[x] = 0;
     work_u[x] = 0;
-    fill[x] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
 }
@@ -50,6 +48,27 @@ __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restric
 // ---------------------------------------------------------------------------
 // Similarity, one wave per undecided edge (GPUSCAN++/GPUScan.cu:174-222)
 // ---------------------------------------------------------------------------
+
+// Dense list of the owned undecided entries (kFlagUDown), in any order, for
+// the one-wave-per-edge kernel: 16 consecutive entries per thread, one
+// append per block iteration.
+__global__ void __launch_bounds__(kBlock) k_list_owned(uint32_t m, const uint8_t* __restrict__ flag,
+                                                       uint32_t* __restrict__ list, uint64_t* __restrict__ cnt) {
+  constexpr uint32_t E = 16;
+  unsigned long long* const ctr[1] = {(unsigned long long*)&cnt[kCntDense]};
+  const uint64_t tile = (uint64_t)kBlock * E;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < m; base += (uint64_t)gridDim.x * tile) {  // block-uniform
+    const uint64_t k0 = base + (uint64_t)threadIdx.x * E;
+    uint32_t sel = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < E; ++j)
+      if (k0 + j < m && flag[k0 + j] == kFlagUDown) sel |= 1u << j;
+    uint32_t pos[1] = {(uint32_t)__popc(sel)};
+    block_reserve<1>(pos, ctr);
+    for (uint32_t j = 0; j < E; ++j)
+      if ((sel >> j) & 1u) list[pos[0]++] = (uint32_t)(k0 + j);
+  }
+}
 
 // One wave per undecided edge (a, b), s = d(a) <= l = d(b): lanes take the
 // short row 64 elements at a time and binary-search each in the long row,
@@ -66,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
                                                             const uint32_t* __restrict__ ulist,
                                                             uint64_t* __restrict__ cnt) {
   const uint32_t lane = lane_id();
-  const uint64_t nU = cnt[kCntUEdges];
+  const uint64_t nU = cnt[kCntDense];
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   uint64_t probes = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < nU; i += waves) {
@@ -134,15 +153,24 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
 
 __global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_t* __restrict__ src,
                                                        const uint8_t* __restrict__ flag, uint32_t* __restrict__ sd) {
+  constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    const bool act = k < m;
-    const uint32_t row = act ? src[k] : 0xFFFFFFFFu;
-    const uint32_t y = act && flag[k] == kFlagY ? 1u : 0u;
-    const uint32_t sum = wave_seg_reduce(row, y, OpAdd());
-    if (wave_seg_head(row) && act && sum) atomicAdd(&sd[row], sum);
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
+       base += waves * kWave * C) {
+    uint32_t row[C], y[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
+      y[j] = k < m && flag[k] == kFlagY ? 1u : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      if (!__ballot(y[j])) continue;
+      const uint32_t sum = wave_seg_reduce(row[j], y[j], OpAdd());
+      if (wave_seg_head(row[j]) && sum) atomicAdd(&sd[row[j]], sum);
+    }
   }
 }
 
@@ -193,11 +221,26 @@ __device__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
 __global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __restrict__ src,
                                                   const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
                                                   const uint8_t* __restrict__ core, int32_t* parent) {
+  constexpr uint32_t C = kEdgeChunks;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
-    const int32_t v = adj[k];
-    const uint32_t u = src[k];
-    if ((uint32_t)v < u && flag[k] == kFlagY && core[u] && core[v]) uf_union(parent, (int32_t)u, v);
+  for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
+    uint32_t u[C];
+    int32_t v[C];
+    bool y[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = k0 + j * stride;
+      y[j] = k < m && flag[k] == kFlagY;
+      u[j] = y[j] ? src[k] : 0u;
+      v[j] = y[j] ? adj[k] : 0;
+      y[j] = y[j] && (uint32_t)v[j] < u[j];
+    }
+    bool cc[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) cc[j] = y[j] && core[u[j]] && core[v[j]];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j)
+      if (cc[j]) uf_union(parent, (int32_t)u[j], v[j]);
   }
 }
 
@@ -216,22 +259,45 @@ __global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* _
 
 __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* __restrict__ src,
                                                    const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
-                                                   const uint8_t* __restrict__ core, int32_t* label) {
+                                                   const uint8_t* __restrict__ core, const uint32_t* __restrict__ rb,
+                                                   const uint32_t* __restrict__ re, int32_t* label,
+                                                   uint64_t* __restrict__ cnt) {
+  constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    const bool act = k < m;
-    uint32_t row = 0xFFFFFFFFu;
-    int32_t cand = kNoLabel;
-    if (act) {
-      row = src[k];
-      const int32_t c = adj[k];
-      if (!core[row] && core[c] && flag[k] == kFlagY) cand = label[c];
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  // entries of core rows, plus the rows of the borders this lane labels
+  // first (views without row offsets count the core rows only)
+  uint32_t labeled = 0;
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
+       base += waves * kWave * C) {
+    uint32_t row[C];
+    int32_t c[C];
+    bool y[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
+      c[j] = k < m ? adj[k] : 0;
+      y[j] = k < m && flag[k] == kFlagY;
     }
-    const int32_t best = wave_seg_reduce(row, cand, OpMin());
-    if (wave_seg_head(row) && act && best != kNoLabel) atomicMin(&label[row], best);
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const bool rc = row[j] != 0xFFFFFFFFu && core[row[j]];
+      labeled += rc;
+      y[j] = y[j] && !rc && core[c[j]];
+    }
+    int32_t cand[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) cand[j] = y[j] ? label[c[j]] : kNoLabel;
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      if (!__ballot(y[j])) continue;
+      const int32_t best = wave_seg_reduce(row[j], cand[j], OpMin());
+      if (wave_seg_head(row[j]) && best != kNoLabel && atomicMin(&label[row[j]], best) == kNoLabel && rb)
+        labeled += re[row[j]] - rb[row[j]];
+    }
   }
+  block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,44 +307,73 @@ __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* _
 // For every vertex still unlabelled: min and max of its neighbours' labels.
 // Two different labels <=> the reference finds a first labelled neighbour and
 // a later one with another label.
-__global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, const uint32_t* __restrict__ src,
+__global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, uint64_t m_all, const uint32_t* __restrict__ src,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ label, int32_t* __restrict__ lo,
-                                                     int32_t* __restrict__ hi) {
+                                                     int32_t* __restrict__ hi, const uint64_t* __restrict__ cnt) {
+  constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    const bool act = k < m;
-    uint32_t row = 0xFFFFFFFFu;
-    int32_t vmin = kNoLabel, vmax = -1;
-    if (act) {
-      row = src[k];
-      if (label[row] == kNoLabel) {
-        const int32_t ly = label[adj[k]];
-        if (ly != kNoLabel) {
-          vmin = ly;
-          vmax = ly;
-        }
-      }
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  // Few entries in labelled rows: walk those rows and hand each label to the
+  // unlabelled neighbours (one gather per entry of a labelled row); else walk
+  // the unlabelled rows and gather their neighbours' labels.  Both give every
+  // unlabelled vertex the min and max label over its neighbours, as long as
+  // every launch over the graph's entries (m_all of them, maybe in several
+  // row chips) takes the same side.
+  const bool from_labeled = 3 * cnt[kCntLabeled] < m_all;
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
+       base += waves * kWave * C) {
+    uint32_t row[C];
+    int32_t nb[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
+      nb[j] = k < m ? adj[k] : 0;
     }
-    vmin = wave_seg_reduce(row, vmin, OpMin());
-    vmax = wave_seg_reduce(row, vmax, OpMax());
-    if (wave_seg_head(row) && act && vmax >= 0) {
-      atomicMin(&lo[row], vmin);
-      atomicMax(&hi[row], vmax);
+    int32_t lr[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) lr[j] = row[j] != 0xFFFFFFFFu ? label[row[j]] : kNoLabel;
+    if (from_labeled) {
+      bool to[C];
+#pragma unroll
+      for (uint32_t j = 0; j < C; ++j) to[j] = lr[j] != kNoLabel && label[nb[j]] == kNoLabel;
+#pragma unroll
+      for (uint32_t j = 0; j < C; ++j)
+        if (to[j]) {
+          atomicMin(&lo[nb[j]], lr[j]);
+          atomicMax(&hi[nb[j]], lr[j]);
+        }
+      continue;
+    }
+    int32_t ly[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) ly[j] = row[j] != 0xFFFFFFFFu && lr[j] == kNoLabel ? label[nb[j]] : kNoLabel;
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const bool has = ly[j] != kNoLabel;
+      if (!__ballot(has)) continue;
+      const int32_t vmin = wave_seg_reduce(row[j], has ? ly[j] : kNoLabel, OpMin());
+      const int32_t vmax = wave_seg_reduce(row[j], has ? ly[j] : -1, OpMax());
+      if (wave_seg_head(row[j]) && vmax >= 0) {
+        atomicMin(&lo[row[j]], vmin);
+        atomicMax(&hi[row[j]], vmax);
+      }
     }
   }
 }
 
+// Also counts the similar entries: sd[x] holds those of row x.
 __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, const int32_t* __restrict__ mu_dev,
-                                                     const int32_t* __restrict__ lo,
+                                                     const uint32_t* __restrict__ sd, const int32_t* __restrict__ lo,
                                                      const int32_t* __restrict__ hi, const uint8_t* __restrict__ core,
                                                      int32_t* __restrict__ label, uint64_t* __restrict__ cnt) {
   if (mu_dev) mu = *mu_dev;
   uint32_t cores = 0, clusters = 0, borders = 0, hubs = 0, outliers = 0;
+  uint64_t similar = 0;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    similar += sd[x];
     int32_t l = label[x];
     if (core[x]) {
       ++cores;
@@ -300,15 +395,9 @@ __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, con
   block_atomic_add((unsigned long long*)&cnt[kCntBorders], borders);
   block_atomic_add((unsigned long long*)&cnt[kCntHubs], hubs);
   block_atomic_add((unsigned long long*)&cnt[kCntOutliers], outliers);
+  block_atomic_add((unsigned long long*)&cnt[kCntSimilar], similar);
 }
 
-__global__ void __launch_bounds__(kBlock) k_count_similar(uint32_t m, const uint8_t* __restrict__ flag,
-                                                          uint64_t* __restrict__ cnt) {
-  uint32_t y = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) y += flag[k] == kFlagY;
-  block_atomic_add((unsigned long long*)&cnt[kCntSimilar], y);
-}
 
 
 // Plain streaming copy (16 bytes per lane per step), the reference point for
@@ -331,12 +420,14 @@ void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
 }
 
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt_u, b.work_u, b.fill, b.cnt);
+  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt_u, b.work_u, b.cnt);
 }
 
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
   if (g->m == 0) return;
   const uint32_t blocks = (uint32_t)(cu_count > 0 ? cu_count : 256) * 8;
+  // the list overwrites the tiered kernels' ulist, which this kernel replaces
+  k_list_owned<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, b.ulist, b.cnt);
   k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag, b.ulist,
                                               b.cnt);
 }
@@ -361,21 +452,21 @@ void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
 }
 
 void launch_border(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  if (g->m) k_border<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.label);
+  if (g->m) k_border<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, g->row_begin,
+                                                     g->row_end, b.label, b.cnt);
 }
 
-void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  if (g->m) k_classify<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.label, b.lo, b.hi);
+void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s, uint64_t m_all) {
+  if (g->m)
+    k_classify<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, m_all ? m_all : g->m, g->src, g->adj, b.label, b.lo,
+                                                         b.hi, b.cnt);
 }
 
 void launch_finalize(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
                      const int32_t* mu_dev) {
-  k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, mu_dev, b.lo, b.hi, b.core, b.label, b.cnt);
+  k_finalize<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, p.mu, mu_dev, b.sd, b.lo, b.hi, b.core, b.label, b.cnt);
 }
 
-void launch_count_similar(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  if (g->m) k_count_similar<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, b.cnt);
-}
 
 void launch_core(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   launch_core_count(g, b, s);
@@ -390,7 +481,6 @@ void launch_cluster(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
 void launch_classify(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   launch_classify_edges(g, b, s);
   launch_finalize(g, b, p, s);
-  launch_count_similar(g, b, s);
 }
 
 }  // namespace sgc

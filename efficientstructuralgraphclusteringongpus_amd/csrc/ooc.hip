@@ -85,7 +85,7 @@ constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
 struct ResidentLayout {
-  size_t deg, rb, re, off_i, off_j, sd, par, lab, lo, hi, core, cntu, work, fill, ust, scan, cnt, bytes;
+  size_t deg, rb, re, off_i, off_j, sd, par, lab, lo, hi, core, cntu, work, cnt, bytes;
 };
 
 ResidentLayout resident_layout(uint32_t n) {
@@ -95,8 +95,7 @@ ResidentLayout resident_layout(uint32_t n) {
   r.off_i = L.take(((size_t)n + 1) * 4), r.off_j = L.take(((size_t)n + 1) * 4);
   r.sd = L.take((size_t)n * 4), r.par = L.take((size_t)n * 4), r.lab = L.take((size_t)n * 4);
   r.lo = L.take((size_t)n * 4), r.hi = L.take((size_t)n * 4), r.core = L.take(n);
-  r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.fill = L.take((size_t)n * 4);
-  r.ust = L.take(((size_t)n + 1) * 4), r.scan = L.take(scan_tmp_words(n) * 4), r.cnt = L.take(kCntSlots * 8);
+  r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.cnt = L.take(kCntSlots * 8);
   r.bytes = L.bytes;
   return r;
 }
@@ -230,9 +229,6 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   pb.core = (uint8_t*)(base + RLay.core);
   pb.cnt_u = (uint32_t*)(base + RLay.cntu);
   pb.work_u = (uint64_t*)(base + RLay.work);
-  pb.fill = (uint32_t*)(base + RLay.fill);
-  pb.ustart = (uint32_t*)(base + RLay.ust);
-  pb.scan_tmp = (uint32_t*)(base + RLay.scan);
   pb.cnt = (uint64_t*)(base + RLay.cnt);
   R.chip = base + resident;
 
@@ -432,8 +428,7 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   SGC_HIP(hipEventRecord(ctx->ev[6], s));
   for (size_t i = 0; i < nrow; ++i) {
     if ((rc = load_rows(nrow - 1 - i, &v))) return rc;
-    launch_classify_edges(&v, rb, s);
-    launch_count_similar(&v, rb, s);
+    launch_classify_edges(&v, rb, s, g->m);
   }
   launch_finalize(&whole, rb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[0], s));

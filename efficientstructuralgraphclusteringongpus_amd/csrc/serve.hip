@@ -179,7 +179,6 @@ void enqueue_query(const Graph* G, const Slot& sl, const ScanParams* p, hipStrea
   launch_border(g, sl.pb, st);
   launch_classify_edges(g, sl.pb, st);
   launch_finalize(g, sl.pb, pv, st, mu_dev);
-  launch_count_similar(g, sl.pb, st);
   (void)hipMemcpyAsync(sl.h_labels, sl.pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, st);
   (void)hipMemcpyAsync(sl.h_cnt, sl.pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, st);
 }

@@ -4,13 +4,13 @@
 //    undecided edge (a, b), a the lower (degree, id) endpoint, is owned by b:
 //    its entry in b's row is marked kFlagUDown, the one in a's row kFlagUUp.
 //    Per owner b: number of owned edges and the sum of their short-row
-//    lengths d(a) (the streaming work).
-// 2. exclusive scan of the owned-edge counts, scatter of the owned entries
-//    into `ulist` grouped by owner.
-// 3. k_plan: each owner's edges are cut into units of bounded work.
+//    lengths d(a) (the streaming work); the owned entries themselves go to
+//    ulist[rb[b] .. rb[b] + count), inside b's own row range, so no scan
+//    over the owners is needed.
+// 2. k_plan: each owner's edges are cut into units of bounded work.
 //    Owners with d(b) <= the tuned cut-off (at most kT1MaxDeg) go to the
 //    small tier, the rest to the workgroup tier.
-// 4. Counting (k_sim_units): the owner's row N(b) is put into LDS and the
+// 3. Counting (k_sim_units): the owner's row N(b) is put into LDS and the
 //    short rows N(a) of the unit's edges are streamed through it in claims
 //    that every wave of the workgroup takes dynamically (up to 16 coalesced
 //    loads per lane in flight in the large tier).  Per edge the running
@@ -19,7 +19,8 @@
 //    The result is written to both directed entries.
 //    Small tier: owners of degree <= 1024, 256-thread workgroups, the row as
 //    a linear-probing hash table.  Large tier: 1024-thread workgroups, the
-//    row one 2^20-id window at a time as an exact 128 KB bitmap.
+//    row as a cuckoo hash (whole, or in runs of 2^20-id windows) or one
+//    window at a time as an exact 128 KB bitmap.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -76,160 +77,84 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
                                                  double eps, bool count_all, PairFilter pf,
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
-                                                 uint64_t* __restrict__ cnt) {
+                                                 uint32_t* __restrict__ ulist, uint64_t* __restrict__ cnt) {
+  constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   uint32_t owned_total = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    bool owned = false;
-    uint32_t row = 0xFFFFFFFFu;  // rows must not decrease across lanes for the segmented reduction
-    uint64_t w = 0;
-    if (k < m) {
-      const uint32_t u = src[k];
-      row = u;
-      const uint32_t v = (uint32_t)adj[k];
-      const uint32_t dv = re[v] - rb[v];
-      uint8_t f = prep_flag(u, v, re[u] - rb[u], dv, eps);
-      if (count_all && f <= kFlagY) {  // every edge to its owner, for exact counts
-        const uint32_t du = re[u] - rb[u];
-        f = (dv > du || (dv == du && v > u)) ? kFlagUUp : kFlagUDown;
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
+       base += waves * kWave * C) {
+    // all loads of the C chunks first, then the gathers they feed
+    uint32_t u[C], v[C], ub[C], du[C], dv[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
+      v[j] = k < m ? (uint32_t)adj[k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const bool in = u[j] != 0xFFFFFFFFu;
+      ub[j] = in ? rb[u[j]] : 0u;
+      du[j] = in ? re[u[j]] : 0u;
+      dv[j] = in ? re[v[j]] - rb[v[j]] : 0u;
+    }
+    uint32_t sb[C], rank[C];
+    uint64_t om[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      const uint32_t row = u[j];
+      bool owned = false;
+      if (k < m) {
+        const uint32_t x = v[j];
+        du[j] -= ub[j];
+        uint8_t f = prep_flag(row, x, du[j], dv[j], eps);
+        if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
+          f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
+        if (pf.on && f == kFlagUDown) {
+          const bool u0 = row - pf.lo0 < pf.hi0 - pf.lo0, u1 = row - pf.lo1 < pf.hi1 - pf.lo1;
+          const bool v0 = x - pf.lo0 < pf.hi0 - pf.lo0, v1 = x - pf.lo1 < pf.hi1 - pf.lo1;
+          if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
+        }
+        flag[k] = f;
+        owned = f == kFlagUDown;
       }
-      if (pf.on && f == kFlagUDown) {
-        const bool u0 = u - pf.lo0 < pf.hi0 - pf.lo0, u1 = u - pf.lo1 < pf.hi1 - pf.lo1;
-        const bool v0 = v - pf.lo0 < pf.hi0 - pf.lo0, v1 = v - pf.lo1 < pf.hi1 - pf.lo1;
-        if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
-      }
-      flag[k] = f;
-      if (f == kFlagUDown) {
-        owned = true;
-        w = dv;
+      om[j] = __ballot(owned);
+      sb[j] = 0;
+      rank[j] = 0;
+      if (om[j]) {
+        if (lane == 0) owned_total += (uint32_t)__popcll(om[j]);
+        // this lane's row segment [h, next head): owned count at the head,
+        // this lane's rank among the owned entries below it
+        const uint64_t ws = wave_seg_reduce(row, owned ? (uint64_t)dv[j] : 0ull, OpAdd());
+        const bool head = wave_seg_head(row);
+        const uint64_t hm = __ballot(head);
+        const uint32_t h = 63 - __clzll((long long)(hm & lanemask_le(lane)));
+        const uint64_t after = hm & ~lanemask_le(lane);
+        const uint64_t seg =
+            (after ? lanemask_lt((uint32_t)(__ffsll((long long)after) - 1)) : ~0ull) & ~lanemask_lt(h);
+        const uint32_t c = (uint32_t)__popcll(om[j] & seg);
+        rank[j] = (uint32_t)__popcll(om[j] & seg & lanemask_lt(lane)) | (h << 24);
+        if (head && c) {
+          sb[j] = atomicAdd(&cnt_u[row], c);
+          atomicAdd((unsigned long long*)&work_u[row], (unsigned long long)ws);
+        }
       }
     }
-    const uint64_t om = __ballot(owned);
-    if (om) {
-      if (lane == 0) owned_total += (uint32_t)__popcll(om);
-      const uint32_t c = wave_seg_reduce(row, owned ? 1u : 0u, OpAdd());
-      const uint64_t ws = wave_seg_reduce(row, w, OpAdd());
-      const bool head = wave_seg_head(row);  // all lanes: the shuffle reads the previous lane
-      if (head && c) {
-        atomicAdd(&cnt_u[row], c);
-        atomicAdd((unsigned long long*)&work_u[row], (unsigned long long)ws);
-      }
+    // owner b's undecided entries go to ulist[rb[b] + slot], slot < cnt_u[b]
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      if (!om[j]) continue;
+      const uint32_t slot = __shfl(sb[j], rank[j] >> 24) + (rank[j] & 0xFFFFFFu);
+      if ((om[j] >> lane) & 1ull) ulist[ub[j] + slot] = (uint32_t)(base + j * kWave + lane);
     }
   }
   block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
 }
 
 // ---------------------------------------------------------------------------
-// 2. Exclusive scan (three kernels) and scatter into ulist
-// ---------------------------------------------------------------------------
-
-constexpr uint32_t kScanItems = 8;
-constexpr uint32_t kScanTile = kBlock * kScanItems;
-
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
-  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
-  const uint32_t incl = wave_incl_scan(v);
-  if (lane == kWave - 1) sh[w] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int i = 0; i < kBlock / kWave; ++i) {
-      const uint32_t t = sh[i];
-      sh[i] = acc;
-      acc += t;
-    }
-    sh[kBlock / kWave] = acc;
-  }
-  __syncthreads();
-  const uint32_t r = sh[w] + incl - v;
-  *total = sh[kBlock / kWave];
-  __syncthreads();
-  return r;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(uint32_t nitems, const uint32_t* __restrict__ in,
-                                                        uint32_t* __restrict__ bsum) {
-  __shared__ uint32_t sh[kBlock / kWave + 1];
-  const uint64_t t0 = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  uint32_t s = 0;
-  for (uint32_t i = 0; i < kScanItems; ++i)
-    if (t0 + i < nitems) s += in[t0 + i];
-  uint32_t total;
-  block_excl_scan(s, sh, &total);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// Single block: exclusive scan of nb block sums in place; out[nitems] gets the
-// grand total.
-__global__ void __launch_bounds__(kBlock) k_scan_blocks(uint32_t nb, uint32_t* __restrict__ bsum, uint32_t nitems,
-                                                        uint32_t* __restrict__ out) {
-  __shared__ uint32_t sh[kBlock / kWave + 1];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kBlock) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < nb ? bsum[i] : 0u;
-    uint32_t total;
-    const uint32_t e = block_excl_scan(v, sh, &total);
-    if (i < nb) bsum[i] = carry + e;
-    carry += total;
-  }
-  if (threadIdx.x == 0) out[nitems] = carry;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_apply(uint32_t nitems, const uint32_t* __restrict__ in,
-                                                       const uint32_t* __restrict__ bsum, uint32_t* __restrict__ out) {
-  __shared__ uint32_t sh[kBlock / kWave + 1];
-  const uint64_t t0 = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
-  for (uint32_t i = 0; i < kScanItems; ++i) {
-    vals[i] = t0 + i < nitems ? in[t0 + i] : 0u;
-    s += vals[i];
-  }
-  uint32_t total;
-  uint32_t run = bsum[blockIdx.x] + block_excl_scan(s, sh, &total);
-  for (uint32_t i = 0; i < kScanItems; ++i) {
-    if (t0 + i < nitems) out[t0 + i] = run;
-    run += vals[i];
-  }
-}
-
-// Owned entries to ulist[ustart[b] + cursor]: one atomic per row segment of
-// each wave; order inside a row is whatever the atomics give (each entry is
-// counted exactly once, which is all the planner needs).
-__global__ void __launch_bounds__(kBlock) k_scatter(uint32_t m, const uint32_t* __restrict__ src,
-                                                    const uint8_t* __restrict__ flag,
-                                                    const uint32_t* __restrict__ cnt_u,
-                                                    const uint32_t* __restrict__ ustart, uint32_t* __restrict__ fill,
-                                                    uint32_t* __restrict__ ulist) {
-  const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < m; base += stride) {
-    const uint64_t k = base + lane;
-    const bool sel = k < m && flag[k] == kFlagUDown;
-    const uint64_t sm = __ballot(sel);
-    if (!sm) continue;
-    const uint32_t row = sel ? src[k] : 0xFFFFFFFFu;
-    const bool head = wave_seg_head(row);
-    const uint64_t hm = __ballot(head);
-    // my segment's head lane and the next head after it
-    const uint32_t h = 63 - __clzll((long long)(hm & lanemask_le(lane)));
-    const uint64_t after = hm & ~lanemask_le(h);
-    const uint32_t next = after ? (uint32_t)(__ffsll((long long)after) - 1) : (uint32_t)kWave;
-    uint32_t segbase = 0;
-    if (sel && head) segbase = atomicAdd(&fill[row], next - h);
-    segbase = __shfl(segbase, h);
-    if (sel) {
-      const uint32_t off = segbase + (lane - h);
-      if (off < cnt_u[row]) ulist[ustart[row] + off] = (uint32_t)k;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 3. Work units
+// 2. Work units
 // ---------------------------------------------------------------------------
 
 // Edges per unit so that a unit streams about `target` short-row elements.
@@ -244,49 +169,58 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                                  const uint32_t* __restrict__ re, SimTuning tune,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
-                                                 const uint32_t* __restrict__ ustart, uint2* __restrict__ units1,
+                                                 uint2* __restrict__ units1,
                                                  uint2* __restrict__ units2, uint64_t* __restrict__ cnt) {
-  const uint32_t lane = lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < n; base += stride) {
-    const uint64_t x = base + lane;
-    uint32_t c1 = 0, c2 = 0, per = 1, owned = 0, start = 0;
-    if (x < n) {
-      owned = cnt_u[x];
-      if (owned) {
-        start = ustart[x];
-        const uint32_t d = re[x] - rb[x];
+  constexpr uint32_t V = 4;  // vertices per thread per block iteration
+  unsigned long long* const ctr[2] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2]};
+  const uint64_t tile = (uint64_t)kBlock * V;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {  // block-uniform
+    uint32_t owned[V], c1[V], c2[V], per[V], start[V];
+#pragma unroll
+    for (uint32_t j = 0; j < V; ++j) {
+      const uint64_t x = base + j * kBlock + threadIdx.x;
+      owned[j] = x < n ? cnt_u[x] : 0u;
+    }
+    uint32_t need[2] = {0, 0};
+#pragma unroll
+    for (uint32_t j = 0; j < V; ++j) {
+      const uint64_t x = base + j * kBlock + threadIdx.x;
+      c1[j] = c2[j] = 0;
+      per[j] = 1;
+      start[j] = 0;
+      if (owned[j]) {
+        start[j] = rb[x];  // the owner's entries sit in ulist at its own row range
+        const uint32_t d = re[x] - start[j];
         if (d <= tune.small_max_degree) {
-          per = edges_per_unit(owned, work_u[x], tune.small_unit_work, kT1MaxEdges);
-          c1 = (owned + per - 1) / per;
+          per[j] = edges_per_unit(owned[j], work_u[x], tune.small_unit_work, kT1MaxEdges);
+          c1[j] = (owned[j] + per[j] - 1) / per[j];
         } else {
-          per = edges_per_unit(owned, work_u[x], tune.large_unit_work, kT2MaxEdges);
-          c2 = (owned + per - 1) / per;
+          per[j] = edges_per_unit(owned[j], work_u[x], tune.large_unit_work, kT2MaxEdges);
+          c2[j] = (owned[j] + per[j] - 1) / per[j];
         }
       }
+      need[0] += c1[j];
+      need[1] += c2[j];
     }
-    const uint32_t i1 = wave_incl_scan(c1), i2 = wave_incl_scan(c2);
-    const uint32_t t1 = __shfl(i1, kWave - 1), t2 = __shfl(i2, kWave - 1);
-    uint32_t b1 = 0, b2 = 0;
-    if (lane == 0) {
-      if (t1) b1 = (uint32_t)atomicAdd((unsigned long long*)&cnt[kCntUnits1], (unsigned long long)t1);
-      if (t2) b2 = (uint32_t)atomicAdd((unsigned long long*)&cnt[kCntUnits2], (unsigned long long)t2);
-    }
-    b1 = __shfl(b1, 0);
-    b2 = __shfl(b2, 0);
-    uint2* dst = c1 ? units1 + b1 + (i1 - c1) : units2 + b2 + (i2 - c2);
-    const uint32_t cu = c1 ? c1 : c2;
-    const uint64_t end = (uint64_t)start + owned;
-    for (uint32_t t = 0; t < cu; ++t) {
-      const uint64_t j0 = (uint64_t)start + (uint64_t)t * per;
-      const uint64_t j1 = j0 + per < end ? j0 + per : end;
-      dst[t] = make_uint2((uint32_t)j0, (uint32_t)j1);
+    block_reserve<2>(need, ctr);
+#pragma unroll
+    for (uint32_t j = 0; j < V; ++j) {
+      uint2* dst = c1[j] ? units1 + need[0] : units2 + need[1];
+      const uint32_t cu = c1[j] + c2[j];
+      const uint64_t end = (uint64_t)start[j] + owned[j];
+      for (uint32_t t = 0; t < cu; ++t) {
+        const uint64_t j0 = (uint64_t)start[j] + (uint64_t)t * per[j];
+        const uint64_t j1 = j0 + per[j] < end ? j0 + per[j] : end;
+        dst[t] = make_uint2((uint32_t)j0, (uint32_t)j1);
+      }
+      need[0] += c1[j];
+      need[1] += c2[j];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// 4. Counting
+// 3. Counting
 // ---------------------------------------------------------------------------
 
 // Ids per bitmap window of the large tier: 2^20 bits = 128 KB of LDS, plus
@@ -835,27 +769,15 @@ void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStr
   if (g->m == 0) return;
   k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, count_all,
                                                    g->pair,
-                                                   b.flag, b.cnt_u, b.work_u, b.cnt);
+                                                   b.flag, b.cnt_u, b.work_u, b.ulist, b.cnt);
 }
-
-static void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t nitems, uint32_t* tmp, hipStream_t s) {
-  const uint32_t nb = (nitems + kScanTile - 1) / kScanTile;
-  const uint32_t grid = nb ? nb : 1;
-  k_scan_reduce<<<grid, kBlock, 0, s>>>(nitems, in, tmp);
-  k_scan_blocks<<<1, kBlock, 0, s>>>(grid, tmp, nitems, out);
-  k_scan_apply<<<grid, kBlock, 0, s>>>(nitems, in, tmp, out);
-}
-
-size_t scan_tmp_words(uint64_t nitems) { return (nitems + kScanTile - 1) / kScanTile + 1; }
 
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m == 0) return;
-  exclusive_scan(b.cnt_u, b.ustart, g->n, b.scan_tmp, s);
-  k_scatter<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, b.flag, b.cnt_u, b.ustart, b.fill, b.ulist);
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
   k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u,
-                                                   b.ustart, b.units1, b.units2, b.cnt);
+                                                   b.units1, b.units2, b.cnt);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
