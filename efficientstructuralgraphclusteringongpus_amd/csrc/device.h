@@ -93,6 +93,9 @@ struct sgc_dgraph {
   uint32_t* row_ptr = nullptr;  // n + 1
   int32_t* adj = nullptr;       // m
   uint32_t* src = nullptr;      // m: row of each entry
+  // m: position of the reverse entry (v, u) of each entry (u, v); resident
+  // graphs only (null: the similarity kernels find it themselves)
+  uint32_t* mirror = nullptr;
   // Row x is adj[row_begin[x], row_end[x]) for every row the similarity
   // kernels read; row_end[x] - row_begin[x] is x's degree for every x.  A
   // resident graph uses row_ptr and row_ptr + 1; an out-of-core chip holds
@@ -122,9 +125,13 @@ struct PassBuffers {
   uint8_t* core;     // n
   uint32_t* cnt_u;   // n: undecided edges owned by each vertex (kFlagUDown entries)
   uint64_t* work_u;  // n: sum of the short-row lengths of those edges
-  uint32_t* ulist;   // m: owner b's undecided entries at [rb[b], rb[b] + cnt_u[b])
-  uint2* units1;     // wave-tier units: [begin, end) ranges of ulist
-  uint2* units2;     // workgroup-tier units
+  // m: owner b's undecided edges at [rb[b], rb[b] + cnt_u[b]) as (entry,
+  // a, row start of a, d(a))
+  uint4* uedge;
+  // ucap work units (begin, end in uedge, owner, owner's row start): small
+  // tier from the front, workgroup tier from the back
+  uint4* units;
+  uint32_t ucap;
   uint64_t* cnt;     // kCntSlots
 };
 
@@ -136,13 +143,15 @@ struct ScanParams {
 // Carves the arrays of one pass over g from the context's work buffer.
 int carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb);
 
-// Units per tier are bounded by the number of owned undecided edges, which is
-// at most m; this is the capacity each unit array is allocated with.
-inline size_t unit_capacity(uint64_t m) { return (size_t)m + 1; }
+// Units of both tiers together are bounded by the number of owned undecided
+// edges: every edge is owned once and has two entries, so at most m / 2.
+inline size_t unit_capacity(uint64_t m) { return (size_t)(m / 2) + 2; }
 
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
+// fills g->mirror (after launch_fill_src); needs symmetric input
+void launch_mirror(const sgc_dgraph* g, hipStream_t s);
 // 2^20-id windows of the large-tier bitmap covering n ids, and the kernel
 // filling g->woff (see sgc_dgraph).
 uint32_t window_count(uint32_t n);

@@ -47,9 +47,8 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   const size_t o_core = take(n);
   const size_t o_cnt_u = take(n * 4);
   const size_t o_work_u = take(n * 8);
-  const size_t o_ulist = take(m * 4);
-  const size_t o_units1 = take(unit_capacity(m) * sizeof(uint2));
-  const size_t o_units2 = take(unit_capacity(m) * sizeof(uint2));
+  const size_t o_uedge = take(m * sizeof(uint4));
+  const size_t o_units = take(unit_capacity(m) * sizeof(uint4));
   int rc = ensure(ctx->work, off);
   if (rc) return rc;
   char* base = (char*)ctx->work.ptr;
@@ -62,9 +61,9 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   pb->core = (uint8_t*)(base + o_core);
   pb->cnt_u = (uint32_t*)(base + o_cnt_u);
   pb->work_u = (uint64_t*)(base + o_work_u);
-  pb->ulist = (uint32_t*)(base + o_ulist);
-  pb->units1 = (uint2*)(base + o_units1);
-  pb->units2 = (uint2*)(base + o_units2);
+  pb->uedge = (uint4*)(base + o_uedge);
+  pb->units = (uint4*)(base + o_units);
+  pb->ucap = (uint32_t)unit_capacity(m);
   pb->cnt = (uint64_t*)ctx->counters.ptr;
   return SGC_OK;
 }
@@ -157,6 +156,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   }
   d->max_degree = (uint32_t)maxd;
   auto cleanup = [d]() {
+    if (d->mirror) (void)hipFree(d->mirror);
     if (d->woff) (void)hipFree(d->woff);
     if (d->row_ptr) (void)hipFree(d->row_ptr);
     if (d->adj) (void)hipFree(d->adj);
@@ -173,6 +173,12 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   }
   d->row_begin = d->row_ptr;
   d->row_end = d->row_ptr + 1;
+  // reverse-entry index: the similarity kernels write both entries of an
+  // edge without searching for the second
+  if (g->m && hipMalloc(&d->mirror, m_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    d->mirror = nullptr;  // the kernels find the reverse entry themselves
+  }
   // window-offset index when there is more than one window and it is no
   // larger than the adjacency itself
   const uint32_t W = window_count(d->n);
@@ -190,6 +196,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
     e = hipMemcpyAsync(d->adj, g->adj, g->m * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
     launch_fill_src(d, ctx->stream);
+    launch_mirror(d, ctx->stream);
     launch_window_offsets(d, ctx->stream);
     e = hipGetLastError();
   }
@@ -206,6 +213,7 @@ extern "C" void sgc_dgraph_free(sgc_dgraph* d) {
   if (!d) return;
   (void)hipSetDevice(d->ctx->device);
   (void)hipStreamSynchronize(d->ctx->stream);
+  if (d->mirror) (void)hipFree(d->mirror);
   if (d->woff) (void)hipFree(d->woff);
   if (d->row_ptr) (void)hipFree(d->row_ptr);
   if (d->adj) (void)hipFree(d->adj);

@@ -27,6 +27,56 @@ __global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t*
   }
 }
 
+// mirror[k] = position of the reverse entry (v, u) of entry k = (u, v), by
+// binary search of u in the sorted row of v (0xFFFFFFFF if the graph lacks
+// it); four searches per thread in lockstep so that their loads overlap.
+__global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
+                                                   uint32_t* __restrict__ mirror) {
+  constexpr uint32_t C = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
+    uint32_t lo[C], hi[C], end[C];
+    int32_t key[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = k0 + j * stride;
+      lo[j] = hi[j] = end[j] = 0;
+      key[j] = 0;
+      if (k < m) {
+        const uint32_t v = (uint32_t)adj[k];
+        key[j] = (int32_t)src[k];
+        lo[j] = rp[v];
+        hi[j] = end[j] = rp[v + 1];
+      }
+    }
+    for (;;) {
+      bool any = false;
+#pragma unroll
+      for (uint32_t j = 0; j < C; ++j) any |= lo[j] < hi[j];
+      if (!any) break;
+      int32_t val[C];
+      uint32_t mid[C];
+#pragma unroll
+      for (uint32_t j = 0; j < C; ++j) {
+        mid[j] = lo[j] + ((hi[j] - lo[j]) >> 1);
+        val[j] = lo[j] < hi[j] ? adj[mid[j]] : 0;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < C; ++j)
+        if (lo[j] < hi[j]) {
+          if (val[j] < key[j])
+            lo[j] = mid[j] + 1;
+          else
+            hi[j] = mid[j];
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j)
+      if (k0 + j * stride < m) mirror[k0 + j * stride] = lo[j] < end[j] && adj[lo[j]] == key[j] ? lo[j] : 0xFFFFFFFFu;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per-pass initialisation
 // ---------------------------------------------------------------------------
@@ -82,14 +132,14 @@ __global__ void __launch_bounds__(kBlock) k_similarity_wave(const uint32_t* __re
                                                             const int32_t* __restrict__ adj,
                                                             const uint32_t* __restrict__ src, double eps,
                                                             uint8_t* __restrict__ flag,
-                                                            const uint32_t* __restrict__ ulist,
+                                                            const uint32_t* __restrict__ list,
                                                             uint64_t* __restrict__ cnt) {
   const uint32_t lane = lane_id();
   const uint64_t nU = cnt[kCntDense];
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   uint64_t probes = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < nU; i += waves) {
-    const uint32_t k = ulist[i];
+    const uint32_t k = list[i];
     const uint32_t b = src[k];
     const uint32_t a = (uint32_t)adj[k];
     const uint32_t ab = rb[a], ae = re[a], bb = rb[b], be = re[b];
@@ -419,6 +469,11 @@ void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
   k_fill_src<<<grid_for(g->n, kBlock / kWave), kBlock, 0, s>>>(g->n, g->row_ptr, g->src);
 }
 
+void launch_mirror(const sgc_dgraph* g, hipStream_t s) {
+  if (g->m && g->mirror)
+    k_mirror<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->src, g->adj, g->mirror);
+}
+
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt_u, b.work_u, b.cnt);
 }
@@ -426,9 +481,10 @@ void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
   if (g->m == 0) return;
   const uint32_t blocks = (uint32_t)(cu_count > 0 ? cu_count : 256) * 8;
-  // the list overwrites the tiered kernels' ulist, which this kernel replaces
-  k_list_owned<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, b.ulist, b.cnt);
-  k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag, b.ulist,
+  // the list takes the tiered kernels' uedge array, which this kernel replaces
+  uint32_t* list = reinterpret_cast<uint32_t*>(b.uedge);
+  k_list_owned<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, b.flag, list, b.cnt);
+  k_similarity_wave<<<blocks, kBlock, 0, s>>>(g->row_begin, g->row_end, g->adj, g->src, p.eps, b.flag, list,
                                               b.cnt);
 }
 

@@ -78,9 +78,9 @@ struct Layout {
   }
 };
 
-// Bytes per entry of the two chip kinds (adj, src, flag, ulist, two unit
-// arrays / adj, src, flag).
-constexpr uint64_t kSimChipBytesPerEntry = 4 + 4 + 1 + 4 + 8 + 8;
+// Bytes per entry of the two chip kinds (adj, src, flag, uedge record, work
+// units at one per two entries / adj, src, flag).
+constexpr uint64_t kSimChipBytesPerEntry = 4 + 4 + 1 + 16 + 8;
 constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
@@ -104,19 +104,17 @@ struct SimChipArrays {
   int32_t* adj;
   uint32_t* src;
   uint8_t* flag;
-  uint32_t* ulist;
-  uint2* units1;
-  uint2* units2;
+  uint4* uedge;
+  uint4* units;
 };
 
 SimChipArrays sim_arrays(char* base, uint64_t cap, size_t* bytes) {
   Layout L;
   const size_t o_adj = L.take(cap * 4), o_src = L.take(cap * 4), o_flag = L.take(cap),
-               o_ulist = L.take(cap * 4), o_u1 = L.take(unit_capacity(cap) * sizeof(uint2)),
-               o_u2 = L.take(unit_capacity(cap) * sizeof(uint2));
+               o_uedge = L.take(cap * sizeof(uint4)), o_units = L.take(unit_capacity(cap) * sizeof(uint4));
   if (bytes) *bytes = L.bytes;
-  return SimChipArrays{(int32_t*)(base + o_adj),    (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
-                       (uint32_t*)(base + o_ulist), (uint2*)(base + o_u1),    (uint2*)(base + o_u2)};
+  return SimChipArrays{(int32_t*)(base + o_adj), (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
+                       (uint4*)(base + o_uedge), (uint4*)(base + o_units)};
 }
 
 size_t sim_chip_bytes(uint64_t cap) {
@@ -333,9 +331,9 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       view.pair.on = true;
       PassBuffers cb = pb;
       cb.flag = A.flag;
-      cb.ulist = A.ulist;
-      cb.units1 = A.units1;
-      cb.units2 = A.units2;
+      cb.uedge = A.uedge;
+      cb.units = A.units;
+      cb.ucap = (uint32_t)unit_capacity(sim_entries);
       const size_t k = chip_no;
       launch_init(&view, cb, s);
       launch_prep(&view, cb, p, s);

@@ -4,8 +4,8 @@
 //    undecided edge (a, b), a the lower (degree, id) endpoint, is owned by b:
 //    its entry in b's row is marked kFlagUDown, the one in a's row kFlagUUp.
 //    Per owner b: number of owned edges and the sum of their short-row
-//    lengths d(a) (the streaming work); the owned entries themselves go to
-//    ulist[rb[b] .. rb[b] + count), inside b's own row range, so no scan
+//    lengths d(a) (the streaming work); the owned edges themselves go to
+//    uedge[rb[b] .. rb[b] + count), inside b's own row range, so no scan
 //    over the owners is needed.
 // 2. k_plan: each owner's edges are cut into units of bounded work.
 //    Owners with d(b) <= the tuned cut-off (at most kT1MaxDeg) go to the
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
                                                  double eps, bool count_all, PairFilter pf,
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
-                                                 uint32_t* __restrict__ ulist, uint64_t* __restrict__ cnt) {
+                                                 uint4* __restrict__ uedge, uint64_t* __restrict__ cnt) {
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
   for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
        base += waves * kWave * C) {
     // all loads of the C chunks first, then the gathers they feed
-    uint32_t u[C], v[C], ub[C], du[C], dv[C];
+    uint32_t u[C], v[C], ub[C], vb[C], du[C], dv[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       const bool in = u[j] != 0xFFFFFFFFu;
       ub[j] = in ? rb[u[j]] : 0u;
       du[j] = in ? re[u[j]] : 0u;
-      dv[j] = in ? re[v[j]] - rb[v[j]] : 0u;
+      vb[j] = in ? rb[v[j]] : 0u;
+      dv[j] = in ? re[v[j]] - vb[j] : 0u;
     }
     uint32_t sb[C], rank[C];
     uint64_t om[C];
@@ -142,12 +143,14 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
         }
       }
     }
-    // owner b's undecided entries go to ulist[rb[b] + slot], slot < cnt_u[b]
+    // owner b's undecided edges go to uedge[rb[b] + slot], slot < cnt_u[b],
+    // as (entry, a, row start of a, d(a))
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       if (!om[j]) continue;
       const uint32_t slot = __shfl(sb[j], rank[j] >> 24) + (rank[j] & 0xFFFFFFu);
-      if ((om[j] >> lane) & 1ull) ulist[ub[j] + slot] = (uint32_t)(base + j * kWave + lane);
+      if ((om[j] >> lane) & 1ull)
+        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane), v[j], vb[j], dv[j]);
     }
   }
   block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
@@ -169,8 +172,8 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                                  const uint32_t* __restrict__ re, SimTuning tune,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
-                                                 uint2* __restrict__ units1,
-                                                 uint2* __restrict__ units2, uint64_t* __restrict__ cnt) {
+                                                 uint4* __restrict__ units, uint32_t ucap,
+                                                 uint64_t* __restrict__ cnt) {
   constexpr uint32_t V = 4;  // vertices per thread per block iteration
   unsigned long long* const ctr[2] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2]};
   const uint64_t tile = (uint64_t)kBlock * V;
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       per[j] = 1;
       start[j] = 0;
       if (owned[j]) {
-        start[j] = rb[x];  // the owner's entries sit in ulist at its own row range
+        start[j] = rb[x];  // the owner's edges sit in uedge at its own row range
         const uint32_t d = re[x] - start[j];
         if (d <= tune.small_max_degree) {
           per[j] = edges_per_unit(owned[j], work_u[x], tune.small_unit_work, kT1MaxEdges);
@@ -203,15 +206,18 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       need[1] += c2[j];
     }
     block_reserve<2>(need, ctr);
+    // small-tier units from the front of `units`, workgroup-tier units from
+    // the back: (begin, end) in uedge, the owner, its row start
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
-      uint2* dst = c1[j] ? units1 + need[0] : units2 + need[1];
       const uint32_t cu = c1[j] + c2[j];
+      const uint32_t x = (uint32_t)(base + j * kBlock + threadIdx.x);
       const uint64_t end = (uint64_t)start[j] + owned[j];
       for (uint32_t t = 0; t < cu; ++t) {
         const uint64_t j0 = (uint64_t)start[j] + (uint64_t)t * per[j];
         const uint64_t j1 = j0 + per[j] < end ? j0 + per[j] : end;
-        dst[t] = make_uint2((uint32_t)j0, (uint32_t)j1);
+        const uint32_t at = c1[j] ? need[0] + t : ucap - 1 - (need[1] + t);
+        units[at] = make_uint4((uint32_t)j0, (uint32_t)j1, x, start[j]);
       }
       need[0] += c1[j];
       need[1] += c2[j];
@@ -403,7 +409,7 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
     const uint32_t pad = NL * kWave - len;
     if (pad && c.wlo == 0 && (c.tab[0] & 1u)) h -= pad;
   } else {
-    record_mirror<NL>(x, len, c.b, s0, mir);
+    if (c.mir_here) record_mirror<NL>(x, len, c.b, s0, mir);
     h = probe_hash<NL>(c.tab, c.mask, c.shift, x, len);
   }
   *used_out = used;
@@ -442,12 +448,11 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
 __global__ void __launch_bounds__(T::kThreads)
-    k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
-                const int32_t* __restrict__ adj,
-                const uint32_t* __restrict__ src, double eps, uint8_t* __restrict__ flag,
+    k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
+                const uint32_t* __restrict__ mirror, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
-                const uint32_t* __restrict__ ulist, const uint2* __restrict__ units, uint64_t* __restrict__ cnt,
-                int units_slot, int queue_slot) {
+                const uint4* __restrict__ uedge, const uint4* __restrict__ units, uint32_t ucap,
+                uint64_t* __restrict__ cnt, int units_slot, int queue_slot) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
   __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
@@ -476,28 +481,32 @@ __global__ void __launch_bounds__(T::kThreads)
   do {              \
   } while (0)
 #endif
-  for (;;) {
-    if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
-    __syncthreads();
-    const uint64_t u = s_unit;
+  for (uint64_t it = 0;; ++it) {
+    // workgroup tier: units from a shared queue (their work varies widely);
+    // small tier: round robin over the many small units, no queue atomics
+    uint64_t u = blockIdx.x + it * gridDim.x;
+    if constexpr (T::kBitmap) {
+      if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
+      __syncthreads();
+      u = s_unit;
+    }
     if (u >= nunits) break;
-    const uint2 un = units[u];
-    const uint32_t ne = un.y - un.x;
-    const uint32_t k0 = ulist[un.x];
-    const uint32_t b = k0 < m ? src[k0] : 0u;
-    const uint32_t bb = rb[b], be = re[b], l = be - bb;
-    if (k0 >= m || ne > T::kMaxEdges || ne == 0 || l == 0) {
+    const uint4 un = units[T::kBitmap ? ucap - 1 - u : u];  // (begin, end) in uedge, owner b, row start of b
+    const uint32_t ne = un.y - un.x, b = un.z, bb = un.w;
+    const uint32_t be = b < n ? re[b] : bb, l = be - bb;
+    if (ne > T::kMaxEdges || ne == 0 || l == 0) {
       __syncthreads();  // never for consistent plans; every thread has read s_unit
       continue;
     }
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
-      const uint32_t k = ulist[un.x + t];
+      const uint4 ed = uedge[un.x + t];  // (entry, a, row start of a, d(a))
+      const uint32_t k = ed.x;
       uint32_t ab = 0, s = 0, cstar = 0, a = 0;
       if (k < m) {
-        a = (uint32_t)adj[k];
-        ab = rb[a];
-        s = re[a] - ab;
+        a = ed.y;
+        ab = ed.z;
+        s = ed.w;
         const double rhs = predicate_rhs(eps, s, l);
         cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
       }
@@ -582,7 +591,9 @@ __global__ void __launch_bounds__(T::kThreads)
       const bool last = kend >= be;
       // slices known from the window-offset index (every pass but a whole-row hash)
       const bool exact = T::kBitmap && woff && !(hashed && whole);
-      const bool mir_here = (hashed && whole) || b - wlo < whi - wlo;  // the table holds id b's range
+      // the table's id range holds b (its position in the short rows is
+      // recorded while streaming when there is no mirror index)
+      const bool mir_here = !mirror && ((hashed && whole) || b - wlo < whi - wlo);
       const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
@@ -688,7 +699,7 @@ __global__ void __launch_bounds__(T::kThreads)
         const uint32_t cstar = e_cstar[t];
         const uint32_t ab = e_ab[t], s = e_s[t];
         const bool sim = cstar != kNone ? hits >= cstar : predicate_exact(hits, predicate_rhs(eps, s, l));
-        uint32_t mpos = e_mir[t];
+        uint32_t mpos = mirror ? mirror[k] : e_mir[t];
         if (mpos == kNone) {
           uint32_t lo = ab, hi = ab + s;
           while (lo < hi) {
@@ -768,16 +779,15 @@ void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
   k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, count_all,
-                                                   g->pair,
-                                                   b.flag, b.cnt_u, b.work_u, b.ulist, b.cnt);
+                                                   g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt);
 }
 
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   if (g->m == 0) return;
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
-  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u,
-                                                   b.units1, b.units2, b.cnt);
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
+                                                   b.ucap, b.cnt);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
@@ -789,12 +799,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
-  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps,
-                                                             b.flag, cn_out, g->woff, g->wcount + 1, b.ulist,
-                                                             b.units2, b.cnt, kCntUnits2, kCntQueue2);
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps, b.flag,
+                                                             cn_out, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                             b.ucap, b.cnt, kCntUnits2, kCntQueue2);
   if (mid) (void)hipEventRecord(mid, s);
-  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->src,
-                                                                 p.eps, b.flag, cn_out, nullptr, 0, b.ulist, b.units1,
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps,
+                                                                 b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
                                                                  b.cnt, kCntUnits1, kCntQueue1);
 }
 
