@@ -35,7 +35,7 @@ constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kT1MaxDeg = 1024;  // small-tier tables are sized for this
-constexpr uint32_t kT1MaxEdges = 256;
+constexpr uint32_t kT1MaxEdges = 192;
 constexpr uint32_t kT2MaxEdges = 512;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t shift) {
@@ -481,15 +481,11 @@ __global__ void __launch_bounds__(T::kThreads)
   do {              \
   } while (0)
 #endif
-  for (uint64_t it = 0;; ++it) {
-    // workgroup tier: units from a shared queue (their work varies widely);
-    // small tier: round robin over the many small units, no queue atomics
-    uint64_t u = blockIdx.x + it * gridDim.x;
-    if constexpr (T::kBitmap) {
-      if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
-      __syncthreads();
-      u = s_unit;
-    }
+  for (;;) {
+    // units from a shared queue: their work varies widely
+    if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
+    __syncthreads();
+    const uint64_t u = s_unit;
     if (u >= nunits) break;
     const uint4 un = units[T::kBitmap ? ucap - 1 - u : u];  // (begin, end) in uedge, owner b, row start of b
     const uint32_t ne = un.y - un.x, b = un.z, bb = un.w;
