@@ -220,7 +220,8 @@ def planted_bench(sgc, ctx, args):
                    "n": g.n, "edges": E, "max_degree": g.max_degree},
         "default_ms_per_step": round(default_s * 1e3, 3),
         "tuning": {"small_max_degree": best.small_max_degree, "small_unit_work": best.small_unit_work,
-                   "large_unit_work": best.large_unit_work, "similarity_ms_tuned": round(best.ms_similarity, 3),
+                   "large_unit_work": best.large_unit_work, "hash_tables": best.hash_tables,
+                   "similarity_ms_tuned": round(best.ms_similarity, 3),
                    "similarity_ms_default": round(best.ms_default, 3), "trials": best.trials},
         "labels_identical_default_vs_tuned": same,
         "stats": {"u_edges": st1.u_edges, "similar_edges": st1.similar_edges, "cores": st1.cores,

@@ -322,30 +322,39 @@ struct ClaimTable {
   const int32_t* adj;
   const uint32_t* tab;
   uint32_t wlo, whi;    // large tier: the window [wlo, whi)
-  uint32_t mask, shift;  // small tier: hash table shape
+  uint32_t mask, shift;  // hash table shape (cuckoo: second table's offset, hash shift)
   uint32_t b;            // the owner
   bool last;             // no later window
   bool mir_here;         // the table's id range holds b
 };
 
-// Workgroup-tier owners of degree <= kCuckooMaxKeys hold their whole row in
-// LDS as a two-table cuckoo hash (2 x 16384 slots, at most half full), and
-// longer rows hold runs of consecutive windows with at most that many keys
-// the same way: every lookup is exactly two LDS reads, and a claim is cut at
-// the run's end instead of at every window's.  Insertion evicts along a
-// chain of at most kCuckooKicks; a failed chain (never observed at this
-// load) makes the unit fall back to bitmap windows.
+// Two-table cuckoo hashes in LDS: tables of 2^bits slots at offsets 0 and
+// 2^bits, hashed by the top `bits` bits of two multiplicative hashes
+// (shift = 32 - bits); every lookup is exactly two LDS reads.
+//   Workgroup tier: 2 x 16384 slots.  Owners of degree <= kCuckooMaxKeys
+//   hold their whole row (at most half full), longer rows hold runs of
+//   consecutive windows with at most that many keys, so a claim is cut at
+//   the run's end instead of at every window's.
+//   Small tier: 2 x 2^bits slots with 2^bits >= 2 d(b) (at most a quarter
+//   full), in place of linear probing, whose lock-step rounds last as long
+//   as the wave's longest probe.
+// Insertion evicts along a chain of at most kCuckooKicks; a failed chain
+// (never observed at these loads) sends the unit to bitmap windows (large
+// tier) or linear probing (small tier).
 constexpr uint32_t kCuckooSlots = 16384;
+constexpr uint32_t kCuckooShift = 18;  // 32 - log2(kCuckooSlots)
 constexpr uint32_t kCuckooMaxKeys = 8192;
 constexpr int kCuckooKicks = 128;
 
-__device__ __forceinline__ uint32_t cuckoo_h0(uint32_t x) { return (x * 0x9E3779B1u) >> 18; }
-__device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x) { return ((x ^ (x >> 16)) * 0x85EBCA77u) >> 18; }
+__device__ __forceinline__ uint32_t cuckoo_h0(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
+__device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x, uint32_t shift) {
+  return ((x ^ (x >> 16)) * 0x85EBCA77u) >> shift;
+}
 
-__device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key) {
+__device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key, uint32_t slots, uint32_t shift) {
   uint32_t k = key, side = 0;
   for (int i = 0; i < kCuckooKicks; ++i) {
-    const uint32_t slot = side ? kCuckooSlots + cuckoo_h1(k) : cuckoo_h0(k);
+    const uint32_t slot = side ? slots + cuckoo_h1(k, shift) : cuckoo_h0(k, shift);
     const uint32_t old = atomicExch(&tab[slot], k);
     if (old == kEmptySlot) return true;
     k = old;  // the evicted key moves to its slot in the other table
@@ -374,8 +383,10 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
     uint32_t v0[NL], v1[NL];
 #pragma unroll
     for (uint32_t j = 0; j < NL; ++j) {
-      v0[j] = c.tab[cuckoo_h0((uint32_t)x[j])];
-      v1[j] = c.tab[kCuckooSlots + cuckoo_h1((uint32_t)x[j])];
+      // workgroup tier: the fixed table shape as constants
+      const uint32_t shift = T::kBitmap ? kCuckooShift : c.shift, second = T::kBitmap ? kCuckooSlots : c.mask;
+      v0[j] = c.tab[cuckoo_h0((uint32_t)x[j], shift)];
+      v1[j] = c.tab[second + cuckoo_h1((uint32_t)x[j], shift)];
     }
 #pragma unroll
     for (uint32_t j = 0; j < NL; ++j) {
@@ -452,7 +463,7 @@ __global__ void __launch_bounds__(T::kThreads)
                 const uint32_t* __restrict__ mirror, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
                 const uint4* __restrict__ uedge, const uint4* __restrict__ units, uint32_t ucap,
-                uint64_t* __restrict__ cnt, int units_slot, int queue_slot) {
+                uint64_t* __restrict__ cnt, int units_slot, int queue_slot, bool hash_tables) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
   __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
@@ -524,7 +535,7 @@ __global__ void __launch_bounds__(T::kThreads)
     // longest run of two or more consecutive windows holding at most
     // kCuckooMaxKeys keys; else the exact bitmap of one window.  A failed
     // cuckoo build sends the rest of the unit to bitmap windows.
-    bool hash_ok = T::kBitmap;
+    bool hash_ok = hash_tables;
     const bool whole = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
     const uint32_t nwin = bw ? wstride - 1 : 0;
     uint32_t kcur = bb;
@@ -555,7 +566,10 @@ __global__ void __launch_bounds__(T::kThreads)
           if (threadIdx.x == 0) s_fail = 0;
           __syncthreads();
           bool ok = true;
-          for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads) ok = cuckoo_insert(tab, (uint32_t)adj[t]) && ok;
+          mask = kCuckooSlots;
+          shift = kCuckooShift;
+          for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads)
+            ok = cuckoo_insert(tab, (uint32_t)adj[t], kCuckooSlots, kCuckooShift) && ok;
           if (!ok) s_fail = 1;
           __syncthreads();
           if (s_fail) {  // redo this pass with bitmap windows
@@ -576,6 +590,24 @@ __global__ void __launch_bounds__(T::kThreads)
             atomicOr(&tab[x >> 5], 1u << (x & 31));
           }
         }
+      } else if (hash_ok) {
+        // two tables of 2^bits >= 2l slots each
+        const uint32_t slots = table_size_for(l);
+        mask = slots;
+        shift = 32 - log2_pow2(slots);
+        for (uint32_t t = threadIdx.x; t < 2 * slots; t += T::kThreads) tab[t] = kEmptySlot;
+        if (threadIdx.x == 0) s_fail = 0;
+        __syncthreads();
+        bool ok = true;
+        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads)
+          ok = cuckoo_insert(tab, (uint32_t)adj[t], slots, shift) && ok;
+        if (!ok) s_fail = 1;
+        __syncthreads();
+        if (s_fail) {  // redo with linear probing
+          hash_ok = false;
+          continue;
+        }
+        hashed = true;
       } else {
         const uint32_t TS = table_size_for(2 * l);
         mask = TS - 1;
@@ -670,8 +702,8 @@ __global__ void __launch_bounds__(T::kThreads)
         uint32_t used;
         const uint32_t nl = (len + kWave - 1) / kWave;
         uint32_t h;
-        if (T::kBitmap && hashed)
-          h = count_dispatch<T, T::kBitmap>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
+        if (hashed)
+          h = count_dispatch<T, true>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
         else
           h = count_dispatch<T, false>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
         if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
@@ -797,11 +829,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps, b.flag,
                                                              cn_out, g->woff, g->wcount + 1, b.uedge, b.units,
-                                                             b.ucap, b.cnt, kCntUnits2, kCntQueue2);
+                                                             b.ucap, b.cnt, kCntUnits2, kCntQueue2,
+                                                             g->tuning.hash_tables != 0);
   if (mid) (void)hipEventRecord(mid, s);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps,
                                                                  b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
-                                                                 b.cnt, kCntUnits1, kCntQueue1);
+                                                                 b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0);
 }
 
 }  // namespace sgc

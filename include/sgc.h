@@ -161,6 +161,9 @@ typedef struct {
   uint32_t small_max_degree; /* owners of degree <= this take the small tier (1..1024; default 1024) */
   double small_unit_work;    /* short-row elements per small-tier work unit (default 65536) */
   double large_unit_work;    /* per workgroup-tier work unit (default 1048576) */
+  uint32_t hash_tables;      /* 1 (default): owner rows as cuckoo hashes where they fit (small tier; whole
+                                rows and runs of 2^20-id windows of <= 8192 keys in the workgroup tier);
+                                0: linear probing (small tier) and one bitmap window at a time only */
   double ms_similarity;      /* sgc_dgraph_autotune: similarity time with these knobs */
   double ms_default;         /* sgc_dgraph_autotune: similarity time with the defaults */
   uint32_t trials;           /* sgc_dgraph_autotune: passes timed */
@@ -168,8 +171,9 @@ typedef struct {
 int sgc_dgraph_get_tuning(const sgc_dgraph* dg, sgc_tuning* t);
 int sgc_dgraph_set_tuning(sgc_dgraph* dg, const sgc_tuning* t);
 /* Times the similarity phase of dg at eps under candidate knobs (coordinate
- * descent over large_unit_work, small_max_degree, small_unit_work; best of
- * two passes each), keeps the fastest on dg and reports it. */
+ * descent over large_unit_work, small_max_degree, small_unit_work,
+ * hash_tables; best of two passes each), keeps the fastest on dg and reports
+ * it. */
 int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* dg, double eps, sgc_tuning* best);
 uint64_t sgc_dgraph_n(const sgc_dgraph* dg);
 uint64_t sgc_dgraph_m(const sgc_dgraph* dg);
