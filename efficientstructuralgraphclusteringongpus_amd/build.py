@@ -49,6 +49,13 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError("build step failed:\n  " + " ".join(cmd) + "\n" + out)
 
 
+# similarity.hip: its atomics are single-lane (the unit queue, claim cursors,
+# per-edge counters).  The AMDGPU atomic optimizer would turn each into a wave
+# reduction that reads the result at once; without it, the queue's next unit
+# is taken while the current one runs.
+FILE_FLAGS = {"similarity.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
+
+
 def _extra_defines() -> list[str]:
     """-D flags from SGC_BUILD_DEFINES (comma separated), for debug builds."""
     return ["-D" + d for d in os.environ.get("SGC_BUILD_DEFINES", "").split(",") if d]
@@ -58,7 +65,7 @@ def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
     if src.endswith(".hip"):
-        _run([HIPCC, *HIP_FLAGS, *_extra_defines(), "-c", path, "-o", obj])
+        _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), *_extra_defines(), "-c", path, "-o", obj])
     else:
         _run([CXX, *HOST_FLAGS, "-c", path, "-o", obj])
     return obj

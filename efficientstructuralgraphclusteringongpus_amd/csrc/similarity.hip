@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
   unsigned long long* const ctr[2] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2]};
   const uint64_t tile = (uint64_t)kBlock * V;
   for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {  // block-uniform
-    uint32_t owned[V], c1[V], c2[V], per[V], start[V];
+    uint32_t owned[V], c1[V], c2[V], per[V], start[V], stop[V];
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint64_t x = base + j * kBlock + threadIdx.x;
@@ -190,10 +190,11 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       const uint64_t x = base + j * kBlock + threadIdx.x;
       c1[j] = c2[j] = 0;
       per[j] = 1;
-      start[j] = 0;
+      start[j] = stop[j] = 0;
       if (owned[j]) {
         start[j] = rb[x];  // the owner's edges sit in uedge at its own row range
-        const uint32_t d = re[x] - start[j];
+        stop[j] = re[x];
+        const uint32_t d = stop[j] - start[j];
         if (d <= tune.small_max_degree) {
           per[j] = edges_per_unit(owned[j], work_u[x], tune.small_unit_work, kT1MaxEdges);
           c1[j] = (owned[j] + per[j] - 1) / per[j];
@@ -207,17 +208,16 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
     }
     block_reserve<2>(need, ctr);
     // small-tier units from the front of `units`, workgroup-tier units from
-    // the back: (begin, end) in uedge, the owner, its row start
+    // the back: (begin, end) in uedge, the owner's row [begin, end) in adj
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint32_t cu = c1[j] + c2[j];
-      const uint32_t x = (uint32_t)(base + j * kBlock + threadIdx.x);
       const uint64_t end = (uint64_t)start[j] + owned[j];
       for (uint32_t t = 0; t < cu; ++t) {
         const uint64_t j0 = (uint64_t)start[j] + (uint64_t)t * per[j];
         const uint64_t j1 = j0 + per[j] < end ? j0 + per[j] : end;
         const uint32_t at = c1[j] ? need[0] + t : ucap - 1 - (need[1] + t);
-        units[at] = make_uint4((uint32_t)j0, (uint32_t)j1, x, start[j]);
+        units[at] = make_uint4((uint32_t)j0, (uint32_t)j1, start[j], stop[j]);
       }
       need[0] += c1[j];
       need[1] += c2[j];
@@ -459,7 +459,7 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
 __global__ void __launch_bounds__(T::kThreads)
-    k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
+    k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ mirror, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
                 const uint4* __restrict__ uedge, const uint4* __restrict__ units, uint32_t ucap,
@@ -470,6 +470,7 @@ __global__ void __launch_bounds__(T::kThreads)
   __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
   __shared__ uint32_t e_cap[T::kMaxEdges], e_a[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend, s_fail;
+  __shared__ uint4 s_next;  // the next unit's descriptor
   // the owner row's window boundaries, from the window-offset index
   constexpr uint32_t kMaxOwnerWindows = 64;
   __shared__ uint32_t s_bw[T::kBitmap ? kMaxOwnerWindows + 1 : 1];
@@ -492,18 +493,43 @@ __global__ void __launch_bounds__(T::kThreads)
   do {              \
   } while (0)
 #endif
+  // Units come from a shared queue (their work varies widely).  Thread 0
+  // takes the next unit while the current one runs and puts its descriptor
+  // into s_next once its own streaming is done, so neither round trip sits
+  // at the top of a unit.
+  uint32_t nxt = 0;
+  auto fetch_next = [&]() {
+    if (nxt < nunits) s_next = units[T::kBitmap ? ucap - 1 - nxt : nxt];
+  };
+  if (threadIdx.x == 0) {
+    nxt = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
+    fetch_next();
+  }
   for (;;) {
-    // units from a shared queue: their work varies widely
-    if (threadIdx.x == 0) s_unit = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
+    if (threadIdx.x == 0) {
+      s_unit = nxt;
+      if (nxt < nunits) nxt = (uint32_t)atomicAdd((unsigned long long*)&cnt[queue_slot], 1ull);
+    }
     __syncthreads();
-    const uint64_t u = s_unit;
+    const uint32_t u = s_unit;
     if (u >= nunits) break;
-    const uint4 un = units[T::kBitmap ? ucap - 1 - u : u];  // (begin, end) in uedge, owner b, row start of b
-    const uint32_t ne = un.y - un.x, b = un.z, bb = un.w;
-    const uint32_t be = b < n ? re[b] : bb, l = be - bb;
+    const uint4 un = s_next;  // (begin, end) in uedge, the owner's row [bb, be) in adj
+    const uint32_t ne = un.y - un.x, bb = un.z, be = un.w, l = be - bb;
     if (ne > T::kMaxEdges || ne == 0 || l == 0) {
-      __syncthreads();  // never for consistent plans; every thread has read s_unit
+      __syncthreads();  // never for consistent plans; every thread has read s_unit and s_next
+      if (threadIdx.x == 0) fetch_next();
       continue;
+    }
+    const uint32_t b = src[bb];
+    // small tier: the owner row in registers, loaded beside the edge records
+    constexpr uint32_t KPT = T::kBitmap ? 1 : (kT1MaxDeg + T::kThreads - 1) / T::kThreads;
+    uint32_t key[KPT];
+    if constexpr (!T::kBitmap) {
+#pragma unroll
+      for (uint32_t j = 0; j < KPT; ++j) {
+        const uint32_t t = threadIdx.x + j * T::kThreads;
+        key[j] = t < l ? (uint32_t)adj[bb + t] : 0u;
+      }
     }
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
@@ -599,8 +625,9 @@ __global__ void __launch_bounds__(T::kThreads)
         if (threadIdx.x == 0) s_fail = 0;
         __syncthreads();
         bool ok = true;
-        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads)
-          ok = cuckoo_insert(tab, (uint32_t)adj[t], slots, shift) && ok;
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; ++j)
+          if (threadIdx.x + j * T::kThreads < l) ok = cuckoo_insert(tab, key[j], slots, shift) && ok;
         if (!ok) s_fail = 1;
         __syncthreads();
         if (s_fail) {  // redo with linear probing
@@ -614,7 +641,9 @@ __global__ void __launch_bounds__(T::kThreads)
         shift = 32 - log2_pow2(TS);
         for (uint32_t t = threadIdx.x; t < TS; t += T::kThreads) tab[t] = kEmptySlot;
         __syncthreads();
-        for (uint32_t t = bb + threadIdx.x; t < be; t += T::kThreads) table_insert(tab, mask, shift, (uint32_t)adj[t]);
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; ++j)
+          if (threadIdx.x + j * T::kThreads < l) table_insert(tab, mask, shift, key[j]);
       }
       const bool last = kend >= be;
       // slices known from the window-offset index (every pass but a whole-row hash)
@@ -711,6 +740,9 @@ __global__ void __launch_bounds__(T::kThreads)
         loaded += len;
         ++claims;
       }
+      // the next unit's descriptor, while the other waves finish streaming
+      // (every thread read s_next before the table barriers)
+      if (threadIdx.x == 0 && kend >= be) fetch_next();
       SGC_TICK(2);
       __syncthreads();
       SGC_TICK(3);
@@ -827,12 +859,12 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
-  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps, b.flag,
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
                                                              cn_out, g->woff, g->wcount + 1, b.uedge, b.units,
                                                              b.ucap, b.cnt, kCntUnits2, kCntQueue2,
                                                              g->tuning.hash_tables != 0);
   if (mid) (void)hipEventRecord(mid, s);
-  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->row_end, g->adj, g->mirror, p.eps,
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
                                                                  b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
                                                                  b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0);
 }
