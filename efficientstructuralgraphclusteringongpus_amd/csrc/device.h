@@ -97,6 +97,9 @@ struct sgc_dgraph {
   // m: position of the reverse entry (v, u) of each entry (u, v); resident
   // graphs only (null: the similarity kernels find it themselves)
   uint32_t* mirror = nullptr;
+  // m: degree of each entry's neighbour, d(adj[k]); resident graphs only
+  // (null: the prep kernel gathers it from the row offsets)
+  uint32_t* ndeg = nullptr;
   // Row x is adj[row_begin[x], row_end[x]) for every row the similarity
   // kernels read; row_end[x] - row_begin[x] is x's degree for every x.  A
   // resident graph uses row_ptr and row_ptr + 1; an out-of-core chip holds
@@ -153,6 +156,8 @@ inline size_t unit_capacity(uint64_t m) { return (size_t)(m / 2) + 2; }
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
 // fills g->mirror (after launch_fill_src); needs symmetric input
 void launch_mirror(const sgc_dgraph* g, hipStream_t s);
+// fills g->ndeg
+void launch_neighbor_degrees(const sgc_dgraph* g, hipStream_t s);
 // 2^20-id windows of the large-tier bitmap covering n ids, and the kernel
 // filling g->woff (see sgc_dgraph).
 uint32_t window_count(uint32_t n);

@@ -157,6 +157,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   d->max_degree = (uint32_t)maxd;
   auto cleanup = [d]() {
     if (d->mirror) (void)hipFree(d->mirror);
+    if (d->ndeg) (void)hipFree(d->ndeg);
     if (d->woff) (void)hipFree(d->woff);
     if (d->row_ptr) (void)hipFree(d->row_ptr);
     if (d->adj) (void)hipFree(d->adj);
@@ -179,6 +180,11 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
     (void)hipGetLastError();
     d->mirror = nullptr;  // the kernels find the reverse entry themselves
   }
+  // neighbour degrees, streamed by the prep kernel on every pass
+  if (g->m && hipMalloc(&d->ndeg, m_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    d->ndeg = nullptr;  // the prep kernel gathers them
+  }
   // window-offset index when there is more than one window and it is no
   // larger than the adjacency itself
   const uint32_t W = window_count(d->n);
@@ -197,6 +203,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   if (e == hipSuccess) {
     launch_fill_src(d, ctx->stream);
     launch_mirror(d, ctx->stream);
+    launch_neighbor_degrees(d, ctx->stream);
     launch_window_offsets(d, ctx->stream);
     e = hipGetLastError();
   }
@@ -214,6 +221,7 @@ extern "C" void sgc_dgraph_free(sgc_dgraph* d) {
   (void)hipSetDevice(d->ctx->device);
   (void)hipStreamSynchronize(d->ctx->stream);
   if (d->mirror) (void)hipFree(d->mirror);
+  if (d->ndeg) (void)hipFree(d->ndeg);
   if (d->woff) (void)hipFree(d->woff);
   if (d->row_ptr) (void)hipFree(d->row_ptr);
   if (d->adj) (void)hipFree(d->adj);

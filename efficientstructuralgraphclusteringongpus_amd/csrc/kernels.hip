@@ -77,6 +77,18 @@ __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* _
   }
 }
 
+// ndeg[k] = d(adj[k]), so the prep kernel streams the neighbour degrees
+// instead of gathering two row offsets per entry on every pass.
+__global__ void __launch_bounds__(kBlock) k_neighbor_degrees(uint32_t m, const uint32_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ adj,
+                                                             uint32_t* __restrict__ ndeg) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
+    const uint32_t v = (uint32_t)adj[k];
+    ndeg[k] = rp[v + 1] - rp[v];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per-pass initialisation
 // ---------------------------------------------------------------------------
@@ -467,6 +479,11 @@ void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s) {
   if (g->n == 0 || g->m == 0) return;
   k_fill_src<<<grid_for(g->n, kBlock / kWave), kBlock, 0, s>>>(g->n, g->row_ptr, g->src);
+}
+
+void launch_neighbor_degrees(const sgc_dgraph* g, hipStream_t s) {
+  if (g->m && g->ndeg)
+    k_neighbor_degrees<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->adj, g->ndeg);
 }
 
 void launch_mirror(const sgc_dgraph* g, hipStream_t s) {

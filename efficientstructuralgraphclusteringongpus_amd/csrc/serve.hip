@@ -338,13 +338,14 @@ extern "C" void sgc_server_destroy(sgc_server* S) {
 extern "C" uint64_t sgc_server_graph_bytes(const sgc_host_graph* g, uint32_t slots_per_graph) {
   if (!g) return 0;
   const uint64_t n = g->n, m = g->m;
-  // row_ptr; adj, src, the reverse-entry index and cn; the window-offset
-  // index (as the upload sizes it); the query slots.  The admission pass's
-  // similarity scratch is the context's and is not held per graph.
+  // row_ptr; adj, src, the reverse-entry and neighbour-degree indexes and
+  // cn; the window-offset index (as the upload sizes it); the query slots.
+  // The admission pass's similarity scratch is the context's and is not
+  // held per graph.
   const uint64_t m_bytes = align256(std::max<uint64_t>(m, 1) * 4);
   const uint32_t W = window_count((uint32_t)n);
   const uint64_t woff = (uint64_t)n * (W + 1) * 4;
-  const uint64_t graph = align256((n + 1) * 4) + 4 * m_bytes +
+  const uint64_t graph = align256((n + 1) * 4) + 5 * m_bytes +
                          (W > 1 && woff <= std::max<uint64_t>(m_bytes, 1ull << 30) ? align256(woff) : 0);
   return graph + slots_per_graph * slot_bytes(n, m);
 }

@@ -73,7 +73,7 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
-                                                 const uint32_t* __restrict__ src,
+                                                 const uint32_t* __restrict__ src, const uint32_t* __restrict__ ndeg,
                                                  double eps, bool count_all, PairFilter pf,
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
@@ -84,21 +84,22 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
   uint32_t owned_total = 0;
   for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
        base += waves * kWave * C) {
-    // all loads of the C chunks first, then the gathers they feed
-    uint32_t u[C], v[C], ub[C], vb[C], du[C], dv[C];
+    // all loads of the C chunks first, then the gathers they feed; with the
+    // neighbour-degree index the neighbour's degree is streamed, not gathered
+    uint32_t u[C], v[C], ub[C], du[C], dv[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
       u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
       v[j] = k < m ? (uint32_t)adj[k] : 0u;
+      dv[j] = k < m && ndeg ? ndeg[k] : 0u;
     }
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const bool in = u[j] != 0xFFFFFFFFu;
       ub[j] = in ? rb[u[j]] : 0u;
       du[j] = in ? re[u[j]] : 0u;
-      vb[j] = in ? rb[v[j]] : 0u;
-      dv[j] = in ? re[v[j]] - vb[j] : 0u;
+      if (!ndeg) dv[j] = in ? re[v[j]] - rb[v[j]] : 0u;
     }
     uint32_t sb[C], rank[C];
     uint64_t om[C];
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       if (!om[j]) continue;
       const uint32_t slot = __shfl(sb[j], rank[j] >> 24) + (rank[j] & 0xFFFFFFu);
       if ((om[j] >> lane) & 1ull)
-        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane), v[j], vb[j], dv[j]);
+        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane), v[j], rb[v[j]], dv[j]);
     }
   }
   block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
@@ -838,7 +839,8 @@ void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
 
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, p.eps, count_all,
+  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, p.eps,
+                                                   count_all,
                                                    g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt);
 }
 
