@@ -53,22 +53,38 @@ def cpu_baseline(g, eps_list, seconds: float):
     rp, adj = g.row_ptr, g.adj
     m = len(adj)
     rng = np.random.default_rng(12345)
-    per_eps = seconds / len(eps_list)
-    total_edges = 0
-    total_t = 0.0
-    for eps in eps_list:
+
+    def timed_sample(eps, budget, nthreads):
         probe = rng.integers(0, m, size=min(m, 20000), dtype=np.uint64)
         t0 = time.perf_counter()
-        ob.orc_similarity_entries(rp, adj, eps, probe, threads)
+        ob.orc_similarity_entries(rp, adj, eps, probe, nthreads)
         dt = max(time.perf_counter() - t0, 1e-6)
-        count = int(min(m, max(20000, len(probe) * per_eps / dt)))
+        count = int(min(m, max(20000, len(probe) * budget / dt)))
         sample = rng.integers(0, m, size=count, dtype=np.uint64)
         t0 = time.perf_counter()
-        ob.orc_similarity_entries(rp, adj, eps, sample, threads)
-        dt = time.perf_counter() - t0
+        ob.orc_similarity_entries(rp, adj, eps, sample, nthreads)
+        return count, time.perf_counter() - t0
+
+    total_edges, total_t = 0, 0.0
+    for eps in eps_list:
+        count, dt = timed_sample(eps, seconds / len(eps_list), threads)
         total_edges += count
         total_t += dt
         log(f"cpu baseline eps={eps}: {count} sampled edges in {dt:.2f} s ({count / dt / 1e6:.2f} M edges/s)")
+    # one thread, a quarter of the budget
+    one_edges, one_t = 0, 0.0
+    for eps in eps_list:
+        count, dt = timed_sample(eps, seconds / 4 / len(eps_list), 1)
+        one_edges += count
+        one_t += dt
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    log(f"cpu baseline: {threads} threads {total_edges / total_t / 1e6:.2f} M edges/s, one thread "
+        f"{one_edges / one_t / 1e6:.2f} M edges/s ({model}, nproc {os.cpu_count()})")
     return {
         "value": total_edges / total_t / 1e9,
         "unit": "GTEPS",
@@ -77,6 +93,9 @@ def cpu_baseline(g, eps_list, seconds: float):
         "sample": (f"oracle prep + merge-intersection similarity of {total_edges} uniformly sampled edges of the "
                    f"same graph over eps in {eps_list} ({total_t:.1f} s, {threads} OpenMP threads); core/"
                    f"union-find/border phases, which are O(m) passes, not included"),
+        "single_thread_value": one_edges / one_t / 1e9,
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
     }
 
 
