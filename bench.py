@@ -240,6 +240,7 @@ def planted_bench(sgc, ctx, args):
         "default_ms_per_step": round(default_s * 1e3, 3),
         "tuning": {"small_max_degree": best.small_max_degree, "small_unit_work": best.small_unit_work,
                    "large_unit_work": best.large_unit_work, "hash_tables": best.hash_tables,
+                   "tiny_work": best.tiny_work,
                    "similarity_ms_tuned": round(best.ms_similarity, 3),
                    "similarity_ms_default": round(best.ms_default, 3), "trials": best.trials},
         "labels_identical_default_vs_tuned": same,

@@ -106,8 +106,8 @@ class QueryStats:
 
 class _Tuning(C.Structure):
     _fields_ = [("small_max_degree", C.c_uint32), ("small_unit_work", C.c_double), ("large_unit_work", C.c_double),
-                ("hash_tables", C.c_uint32), ("ms_similarity", C.c_double), ("ms_default", C.c_double),
-                ("trials", C.c_uint32)]
+                ("hash_tables", C.c_uint32), ("tiny_work", C.c_uint32), ("ms_similarity", C.c_double),
+                ("ms_default", C.c_double), ("trials", C.c_uint32)]
 
 
 @dataclass
@@ -116,6 +116,7 @@ class Tuning:
     small_unit_work: float = 65536.0
     large_unit_work: float = 1048576.0
     hash_tables: int = 1
+    tiny_work: int = 512
     ms_similarity: float = 0.0
     ms_default: float = 0.0
     trials: int = 0
@@ -450,8 +451,8 @@ class DeviceGraph:
 
     @tuning.setter
     def tuning(self, value: Tuning) -> None:
-        t = _Tuning(value.small_max_degree, value.small_unit_work, value.large_unit_work, value.hash_tables, 0.0, 0.0,
-                    0)
+        t = _Tuning(value.small_max_degree, value.small_unit_work, value.large_unit_work, value.hash_tables,
+                    value.tiny_work, 0.0, 0.0, 0)
         _check(self._lib.sgc_dgraph_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, eps: float) -> Tuning:

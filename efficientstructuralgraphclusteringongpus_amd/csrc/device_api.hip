@@ -237,6 +237,7 @@ extern "C" int sgc_dgraph_get_tuning(const sgc_dgraph* d, sgc_tuning* t) {
   t->small_unit_work = d->tuning.small_unit_work;
   t->large_unit_work = d->tuning.large_unit_work;
   t->hash_tables = d->tuning.hash_tables;
+  t->tiny_work = d->tuning.tiny_work;
   return SGC_OK;
 }
 
@@ -244,12 +245,14 @@ extern "C" int sgc_dgraph_set_tuning(sgc_dgraph* d, const sgc_tuning* t) {
   clear_error();
   if (!d || !t) return fail(SGC_EINVAL, "null argument");
   if (t->small_max_degree < 1 || t->small_max_degree > 1024 || !(t->small_unit_work >= 64.0) ||
-      !(t->large_unit_work >= 64.0) || t->hash_tables > 1)
-    return fail(SGC_EINVAL, "tuning out of range (small_max_degree 1..1024, unit work >= 64, hash_tables 0/1)");
+      !(t->large_unit_work >= 64.0) || t->hash_tables > 1 || t->tiny_work > 65536)
+    return fail(SGC_EINVAL,
+                "tuning out of range (small_max_degree 1..1024, unit work >= 64, hash_tables 0/1, tiny_work <= 65536)");
   d->tuning.small_max_degree = t->small_max_degree;
   d->tuning.small_unit_work = t->small_unit_work;
   d->tuning.large_unit_work = t->large_unit_work;
   d->tuning.hash_tables = t->hash_tables;
+  d->tuning.tiny_work = t->tiny_work;
   return SGC_OK;
 }
 
@@ -396,6 +399,15 @@ extern "C" int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* g, double eps, sgc_
     if ((rc = measure(t, &ms))) return rc;
     if (ms < cur_ms) cur = t, cur_ms = ms;
   }
+  const uint32_t tiny_work[] = {0, 256, 512, 2048};
+  for (uint32_t v : tiny_work) {
+    SimTuning t = cur;
+    t.tiny_work = v;
+    double ms;
+    if (v == cur.tiny_work) continue;
+    if ((rc = measure(t, &ms))) return rc;
+    if (ms < cur_ms) cur = t, cur_ms = ms;
+  }
   g->tuning = cur;
   if (best) {
     memset(best, 0, sizeof(*best));
@@ -403,6 +415,7 @@ extern "C" int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* g, double eps, sgc_
     best->small_unit_work = cur.small_unit_work;
     best->large_unit_work = cur.large_unit_work;
     best->hash_tables = cur.hash_tables;
+    best->tiny_work = cur.tiny_work;
     best->ms_similarity = cur_ms;
     best->ms_default = ms_default;
     best->trials = trials;

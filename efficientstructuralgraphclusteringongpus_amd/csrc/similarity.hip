@@ -161,6 +161,14 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
 // 2. Work units
 // ---------------------------------------------------------------------------
 
+// Tiny owners: the merge work of all their edges, sum of d(a) + d(b), is at
+// most tiny_work.  They get no work unit; k_sim_tiny decides their edges with
+// one thread per owner, since a unit's fixed set-up would cost more than
+// their whole work.
+__device__ __forceinline__ bool tiny_owner(uint32_t owned, uint64_t work, uint32_t l, uint32_t tiny_work) {
+  return work + (uint64_t)owned * l <= tiny_work;
+}
+
 // Edges per unit so that a unit streams about `target` short-row elements.
 __device__ __forceinline__ uint32_t edges_per_unit(uint32_t cnt, uint64_t work, double target, uint32_t cap) {
   double k = work ? target * (double)cnt / (double)work : (double)cnt;
@@ -196,7 +204,9 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
         start[j] = rb[x];  // the owner's edges sit in uedge at its own row range
         stop[j] = re[x];
         const uint32_t d = stop[j] - start[j];
-        if (d <= tune.small_max_degree) {
+        if (tiny_owner(owned[j], work_u[x], d, tune.tiny_work)) {
+          // decided by k_sim_tiny
+        } else if (d <= tune.small_max_degree) {
           per[j] = edges_per_unit(owned[j], work_u[x], tune.small_unit_work, kT1MaxEdges);
           c1[j] = (owned[j] + per[j] - 1) / per[j];
         } else {
@@ -222,6 +232,73 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       }
       need[0] += c1[j];
       need[1] += c2[j];
+    }
+  }
+}
+
+// Tiny owners (tiny_owner): one thread per owner merges the short row and the
+// owner row of each of its undecided edges in turn, with the same exact
+// threshold and early exit as the tiered kernels, and writes both entries.
+__global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t n, uint32_t m, const uint32_t* __restrict__ rb,
+                                                     const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
+                                                     const uint32_t* __restrict__ mirror, double eps,
+                                                     uint32_t tiny_work, const uint32_t* __restrict__ cnt_u,
+                                                     const uint64_t* __restrict__ work_u,
+                                                     const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
+                                                     uint32_t* __restrict__ cn_out) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t xi = (uint64_t)blockIdx.x * kBlock + threadIdx.x; xi < n; xi += stride) {
+    const uint32_t x = (uint32_t)xi, owned = cnt_u[x];
+    if (!owned) continue;
+    const uint32_t bb = rb[x], be = re[x], l = be - bb;
+    if (!tiny_owner(owned, work_u[x], l, tiny_work)) continue;
+    for (uint32_t i = 0; i < owned; ++i) {
+      const uint4 ed = uedge[bb + i];  // (entry, a, row start of a, d(a))
+      const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
+      if (k >= m) continue;
+      const double rhs = predicate_rhs(eps, s, l);
+      const uint32_t cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
+      uint32_t ia = ab, ib = bb, cn = 0, mpos = kNone;
+      while (ia < ae && ib < be) {
+        if (cstar != kNone) {
+          const uint32_t rest = ae - ia < be - ib ? ae - ia : be - ib;
+          if (cn >= cstar || cn + rest < cstar) break;
+        }
+        const int32_t va = adj[ia], vb = adj[ib];
+        if (va == (int32_t)x) mpos = ia;
+        if (va < vb) {
+          ++ia;
+        } else if (va > vb) {
+          ++ib;
+        } else {
+          ++cn;
+          ++ia;
+          ++ib;
+        }
+      }
+      const bool sim = cstar != kNone ? cn >= cstar : predicate_exact(cn, rhs);
+      if (mirror) {
+        mpos = mirror[k];
+      } else if (mpos == kNone) {
+        // the merge stopped before reaching x in a's row
+        uint32_t lo = ia, hi = ae;
+        while (lo < hi) {
+          const uint32_t mid = lo + ((hi - lo) >> 1);
+          if (adj[mid] < (int32_t)x)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        mpos = lo < ae && adj[lo] == (int32_t)x ? lo : kNone;
+      }
+      if (cn_out) {
+        cn_out[k] = cn;
+        if (mpos != kNone) cn_out[mpos] = cn;
+      } else {
+        const uint8_t f = sim ? kFlagY : kFlagN;
+        flag[k] = f;
+        if (mpos != kNone) flag[mpos] = f;
+      }
     }
   }
 }
@@ -869,6 +946,10 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
                                                                  b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
                                                                  b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0);
+  if (g->tuning.tiny_work)
+    k_sim_tiny<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->mirror, p.eps,
+                                                         g->tuning.tiny_work, b.cnt_u, b.work_u, b.uedge, b.flag,
+                                                         cn_out);
 }
 
 }  // namespace sgc

@@ -164,6 +164,9 @@ typedef struct {
   uint32_t hash_tables;      /* 1 (default): owner rows as cuckoo hashes where they fit (small tier; whole
                                 rows and runs of 2^20-id windows of <= 8192 keys in the workgroup tier);
                                 0: linear probing (small tier) and one bitmap window at a time only */
+  uint32_t tiny_work;        /* owners whose edges' merge work, sum of d(a) + d(b), is at most this are
+                                decided by one thread each, without a work unit (0..65536; default 512;
+                                0 turns this off) */
   double ms_similarity;      /* sgc_dgraph_autotune: similarity time with these knobs */
   double ms_default;         /* sgc_dgraph_autotune: similarity time with the defaults */
   uint32_t trials;           /* sgc_dgraph_autotune: passes timed */
@@ -172,8 +175,8 @@ int sgc_dgraph_get_tuning(const sgc_dgraph* dg, sgc_tuning* t);
 int sgc_dgraph_set_tuning(sgc_dgraph* dg, const sgc_tuning* t);
 /* Times the similarity phase of dg at eps under candidate knobs (coordinate
  * descent over large_unit_work, small_max_degree, small_unit_work,
- * hash_tables; best of two passes each), keeps the fastest on dg and reports
- * it. */
+ * hash_tables, tiny_work; best of two passes each), keeps the fastest on dg
+ * and reports it. */
 int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* dg, double eps, sgc_tuning* best);
 uint64_t sgc_dgraph_n(const sgc_dgraph* dg);
 uint64_t sgc_dgraph_m(const sgc_dgraph* dg);

@@ -15,8 +15,8 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-@pytest.mark.parametrize("knobs", [(128, 16384.0, 262144.0, 1), (512, 262144.0, 4194304.0, 0),
-                                   (1024, 4096.0, 65536.0, 0), (1, 64.0, 64.0, 1)])
+@pytest.mark.parametrize("knobs", [(128, 16384.0, 262144.0, 1, 512), (512, 262144.0, 4194304.0, 0, 0),
+                                   (1024, 4096.0, 65536.0, 0, 65536), (1, 64.0, 64.0, 1, 2048)])
 def test_any_knobs_give_identical_results(ctx, knobs):
     say("knobs", knobs)
     g = sgc.HostGraph.rmat(16, 16, seed=21)
@@ -25,7 +25,7 @@ def test_any_knobs_give_identical_results(ctx, knobs):
         ref = sgc.scan(ctx, dg, 0.3, 3, want_sim=True)
         dg.tuning = sgc.Tuning(*knobs)
         t = dg.tuning
-        assert (t.small_max_degree, t.small_unit_work, t.large_unit_work, t.hash_tables) == knobs
+        assert (t.small_max_degree, t.small_unit_work, t.large_unit_work, t.hash_tables, t.tiny_work) == knobs
         r = sgc.scan(ctx, dg, 0.3, 3, want_sim=True)
         assert np.array_equal(r.sim, ref.sim) and np.array_equal(r.labels, ref.labels)
     finally:
@@ -35,13 +35,17 @@ def test_any_knobs_give_identical_results(ctx, knobs):
 @pytest.mark.parametrize("eps", [0.2, 0.5])
 def test_hash_tables_off_over_several_windows(ctx, eps):
     """n > 2^20: the workgroup tier's cuckoo hashes (whole rows, window runs)
-    and the small tier's against bitmap windows and linear probing only."""
+    and the small tier's against bitmap windows and linear probing only, and
+    tiny owners merged by one thread against their own work units."""
     say("hash tables off", eps)
     g = sgc.HostGraph.rmat(21, 16, seed=12)
     dg = sgc.DeviceGraph(ctx, g)
     try:
         ref = sgc.scan(ctx, dg, eps, 3, want_sim=True)
         dg.tuning = sgc.Tuning(hash_tables=0)
+        r = sgc.scan(ctx, dg, eps, 3, want_sim=True)
+        assert np.array_equal(r.sim, ref.sim) and np.array_equal(r.labels, ref.labels)
+        dg.tuning = sgc.Tuning(tiny_work=0)
         r = sgc.scan(ctx, dg, eps, 3, want_sim=True)
         assert np.array_equal(r.sim, ref.sim) and np.array_equal(r.labels, ref.labels)
     finally:
@@ -60,6 +64,7 @@ def test_autotune_keeps_a_candidate_and_results(ctx):
         assert best.small_unit_work in (16384.0, 65536.0, 262144.0)
         assert best.large_unit_work in (262144.0, 1048576.0, 4194304.0)
         assert best.hash_tables in (0, 1)
+        assert best.tiny_work in (0, 256, 512, 2048)
         assert best.trials >= 12 and best.ms_similarity <= best.ms_default
         t = dg.tuning
         assert (t.small_max_degree, t.large_unit_work) == (best.small_max_degree, best.large_unit_work)
@@ -74,7 +79,7 @@ def test_tuning_rejects_out_of_range_knobs(ctx):
     dg = sgc.DeviceGraph(ctx, g)
     try:
         for bad in ((0, 65536.0, 1048576.0), (2048, 65536.0, 1048576.0), (512, 1.0, 1048576.0),
-                    (512, 65536.0, 1048576.0, 2)):
+                    (512, 65536.0, 1048576.0, 2), (512, 65536.0, 1048576.0, 1, 70000)):
             with pytest.raises(sgc.SGCError):
                 dg.tuning = sgc.Tuning(*bad)
     finally:
