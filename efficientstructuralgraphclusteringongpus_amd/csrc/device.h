@@ -31,9 +31,9 @@ enum Counter {
   kCntHubs,
   kCntOutliers,
   kCntProbes,
-  kCntUnits1,   // wave-tier work units
+  kCntUnits1,   // small-tier work units
   kCntUnits2,   // workgroup-tier work units
-  kCntQueue1,   // next wave-tier unit to claim
+  kCntQueue1,   // next small-tier unit to claim
   kCntQueue2,   // next workgroup-tier unit to claim
   kCntLoaded,   // short-row elements loaded by the claims (consumed or not)
   kCntClaims,   // claims taken
