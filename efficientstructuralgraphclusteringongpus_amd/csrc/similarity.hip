@@ -328,10 +328,13 @@ struct Tier {
 using SmallTier = Tier<256, 4 * kT1MaxDeg, false, kT1MaxEdges, 256>;
 using LargeTier = Tier<1024, kWindowWords + 4, true, kT2MaxEdges, 1024>;
 
-// Elements to claim next for an edge in state (hits, consumed): the fewest
-// after which a decision can be possible, min(c* - hits, hits + rest - c* + 1),
-// rounded up to whole wave loads and capped at one chunk.  Rows counted fully
-// (c* = kNone) claim whole chunks.
+// Elements to claim next for an edge in state (hits, consumed): twice the
+// fewest after which a decision can be possible, min(c* - hits, hits + rest -
+// c* + 1), but at most 256 more than that, rounded up to whole wave loads and
+// capped at one chunk.  A decision at the earliest possible point needs every
+// claimed element to hit (or to miss), which is rare, so the extra elements
+// are seldom wasted and keep more loads in flight per claim.  Rows counted
+// fully (c* = kNone) claim whole chunks.
 __device__ __forceinline__ uint32_t claim_size(unsigned long long st, uint32_t cstar, uint32_t s, uint32_t chunk) {
   if (cstar == kNone) return chunk;
   const uint32_t hits = (uint32_t)(st >> 32), used = (uint32_t)st;
@@ -339,6 +342,7 @@ __device__ __forceinline__ uint32_t claim_size(unsigned long long st, uint32_t c
   const uint32_t to_reach = cstar - hits;              // hits < c* while undecided
   const uint32_t to_fail = hits + rest - cstar + 1;    // hits + rest >= c* while undecided
   uint32_t want = to_reach < to_fail ? to_reach : to_fail;
+  want = want < 256 ? 2 * want : want + 256;
   want = (want + kWave - 1) / kWave * kWave;
   return want < chunk ? want : chunk;
 }
