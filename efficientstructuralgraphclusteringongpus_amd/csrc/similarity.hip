@@ -258,23 +258,33 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t n, uint32_t m, con
       if (k >= m) continue;
       const double rhs = predicate_rhs(eps, s, l);
       const uint32_t cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
+      // merge in blocks of 4 entries per row: count the matches between the
+      // two current blocks, then advance the block with the smaller last
+      // value (both if equal).  A matching pair is current together exactly
+      // once, and cn + min(entries left in either row) bounds the final
+      // count, so the early exit is the tiered kernels'.
       uint32_t ia = ab, ib = bb, cn = 0, mpos = kNone;
       while (ia < ae && ib < be) {
         if (cstar != kNone) {
           const uint32_t rest = ae - ia < be - ib ? ae - ia : be - ib;
           if (cn >= cstar || cn + rest < cstar) break;
         }
-        const int32_t va = adj[ia], vb = adj[ib];
-        if (va == (int32_t)x) mpos = ia;
-        if (va < vb) {
-          ++ia;
-        } else if (va > vb) {
-          ++ib;
-        } else {
-          ++cn;
-          ++ia;
-          ++ib;
+        const uint32_t na = ae - ia < 4 ? ae - ia : 4, nb = be - ib < 4 ? be - ib : 4;
+        int32_t va[4], vb[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          va[q] = q < na ? adj[ia + q] : 0;
+          vb[q] = q < nb ? adj[ib + q] : 0;
         }
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          if (q < na && va[q] == (int32_t)x) mpos = ia + q;
+#pragma unroll
+          for (uint32_t r = 0; r < 4; ++r) cn += (q < na && r < nb && va[q] == vb[r]) ? 1u : 0u;
+        }
+        const int32_t amax = va[na - 1], bmax = vb[nb - 1];
+        if (amax <= bmax) ia += na;
+        if (bmax <= amax) ib += nb;
       }
       const bool sim = cstar != kNone ? cn >= cstar : predicate_exact(cn, rhs);
       if (mirror) {
