@@ -74,6 +74,7 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                  const uint32_t* __restrict__ src, const uint32_t* __restrict__ ndeg,
+                                                 const uint32_t* __restrict__ mirror,
                                                  double eps, bool count_all, PairFilter pf,
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
@@ -145,13 +146,15 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       }
     }
     // owner b's undecided edges go to uedge[rb[b] + slot], slot < cnt_u[b],
-    // as (entry, a, row start of a, d(a))
+    // as (entry, its reverse entry or kNone without the index, row start of
+    // a, d(a))
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       if (!om[j]) continue;
       const uint32_t slot = __shfl(sb[j], rank[j] >> 24) + (rank[j] & 0xFFFFFFu);
       if ((om[j] >> lane) & 1ull)
-        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane), v[j], rb[v[j]], dv[j]);
+        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane),
+                                         mirror ? mirror[base + j * kWave + lane] : kNone, rb[v[j]], dv[j]);
     }
   }
   block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
@@ -253,7 +256,7 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t n, uint32_t m, con
     const uint32_t bb = rb[x], be = re[x], l = be - bb;
     if (!tiny_owner(owned, work_u[x], l, tiny_work)) continue;
     for (uint32_t i = 0; i < owned; ++i) {
-      const uint4 ed = uedge[bb + i];  // (entry, a, row start of a, d(a))
+      const uint4 ed = uedge[bb + i];  // (entry, reverse entry, row start of a, d(a))
       const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
       if (k >= m) continue;
       const double rhs = predicate_rhs(eps, s, l);
@@ -288,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t n, uint32_t m, con
       }
       const bool sim = cstar != kNone ? cn >= cstar : predicate_exact(cn, rhs);
       if (mirror) {
-        mpos = mirror[k];
+        mpos = ed.y;
       } else if (mpos == kNone) {
         // the merge stopped before reaching x in a's row
         uint32_t lo = ia, hi = ae;
@@ -625,11 +628,11 @@ __global__ void __launch_bounds__(T::kThreads)
     }
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
-      const uint4 ed = uedge[un.x + t];  // (entry, a, row start of a, d(a))
+      const uint4 ed = uedge[un.x + t];  // (entry, reverse entry, row start of a, d(a))
       const uint32_t k = ed.x;
       uint32_t ab = 0, s = 0, cstar = 0, a = 0;
       if (k < m) {
-        a = ed.y;
+        if (T::kBitmap && woff) a = (uint32_t)adj[k];  // for the window-offset index
         ab = ed.z;
         s = ed.w;
         const double rhs = predicate_rhs(eps, s, l);
@@ -641,7 +644,7 @@ __global__ void __launch_bounds__(T::kThreads)
       e_s[t] = s;
       e_cur[t] = ab;
       e_cstar[t] = cstar;
-      e_mir[t] = kNone;
+      e_mir[t] = k < m ? ed.y : kNone;
       e_state[t] = 0ull;
     }
     if (bw && threadIdx.x < wstride) s_bw[threadIdx.x] = woff[(size_t)b * wstride + threadIdx.x];
@@ -851,8 +854,8 @@ __global__ void __launch_bounds__(T::kThreads)
         const uint32_t cstar = e_cstar[t];
         const uint32_t ab = e_ab[t], s = e_s[t];
         const bool sim = cstar != kNone ? hits >= cstar : predicate_exact(hits, predicate_rhs(eps, s, l));
-        uint32_t mpos = mirror ? mirror[k] : e_mir[t];
-        if (mpos == kNone) {
+        uint32_t mpos = e_mir[t];
+        if (mpos == kNone && !mirror) {
           uint32_t lo = ab, hi = ab + s;
           while (lo < hi) {
             const uint32_t mid = lo + ((hi - lo) >> 1);
@@ -930,7 +933,8 @@ void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
 
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, p.eps,
+  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror,
+                                                   p.eps,
                                                    count_all,
                                                    g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt);
 }
