@@ -337,6 +337,9 @@ struct Tier {
   static constexpr uint32_t kMaxEdges = MAX_EDGES;
   static constexpr uint32_t kChunk = CHUNK;
   static constexpr uint32_t kLoads = CHUNK / kWave;
+  // waves per SIMD the register allocation must allow: six small-tier
+  // workgroups per CU fit its LDS
+  static constexpr int kMinWaves = BITMAP ? 1 : 6;
 };
 using SmallTier = Tier<256, 4 * kT1MaxDeg, false, kT1MaxEdges, 256>;
 using LargeTier = Tier<1024, kWindowWords + 4, true, kT2MaxEdges, 1024>;
@@ -553,7 +556,7 @@ __device__ __forceinline__ uint32_t count_dispatch(uint32_t nl, const ClaimTable
 // the claim's filled 64-element loads.  Rows too long for the threshold form
 // (s > kMaxThresholdRow) are counted fully.
 template <class T>
-__global__ void __launch_bounds__(T::kThreads)
+__global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
     k_sim_units(uint32_t n, uint32_t m, const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
                 const uint32_t* __restrict__ mirror, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
