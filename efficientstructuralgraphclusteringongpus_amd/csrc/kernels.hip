@@ -224,14 +224,21 @@ __global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
-      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
       y[j] = k < m && flag[k] == kFlagY ? 1u : 0u;
     }
+    // rows of similar entries only; the others get a marker row, so one
+    // row's similar entries may form several runs of lanes: each run's first
+    // lane adds the run's count (ballot over [first lane, next run))
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) row[j] = y[j] ? src[base + j * kWave + lane] : 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
-      if (!__ballot(y[j])) continue;
-      const uint32_t sum = wave_seg_reduce(row[j], y[j], OpAdd());
-      if (wave_seg_head(row[j]) && sum) atomicAdd(&sd[row[j]], sum);
+      const uint64_t ym = __ballot(y[j]);
+      if (!ym) continue;
+      const bool head = wave_seg_head(row[j]);
+      const uint64_t after = __ballot(head) & ~lanemask_le(lane);
+      const uint64_t run = (after ? lanemask_lt((uint32_t)(__ffsll((long long)after) - 1)) : ~0ull) & ~lanemask_lt(lane);
+      if (head && y[j]) atomicAdd(&sd[row[j]], (uint32_t)__popcll(ym & run));
     }
   }
 }
@@ -241,18 +248,26 @@ __global__ void __launch_bounds__(kBlock) k_core_count(uint32_t m, const uint32_
 // similar edges, ed only discounts dissimilar ones, and every edge of a
 // still-undecided vertex is evaluated.
 // mu comes from mu_dev when given (parameters of a captured launch), else mu.
+// Also adds the entries of the core rows to cnt[kCntLabeled] when the row
+// offsets are given (rb null: views without them count nothing here).
 __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t* __restrict__ sd, int32_t mu,
-                                                      const int32_t* __restrict__ mu_dev, uint8_t* __restrict__ core,
-                                                      int32_t* __restrict__ parent, int32_t* __restrict__ label) {
+                                                      const int32_t* __restrict__ mu_dev,
+                                                      const uint32_t* __restrict__ rb,
+                                                      const uint32_t* __restrict__ re, uint8_t* __restrict__ core,
+                                                      int32_t* __restrict__ parent, int32_t* __restrict__ label,
+                                                      uint64_t* __restrict__ cnt) {
   if (mu_dev) mu = *mu_dev;
   const uint32_t thr = mu >= 1 ? (uint32_t)mu : 1u;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  uint64_t labeled = 0;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     const bool c = sd[x] >= thr;
     core[x] = c;
     parent[x] = (int32_t)x;
     label[x] = kNoLabel;
+    if (c && rb) labeled += re[x] - rb[x];
   }
+  block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
 }
 
 // ---------------------------------------------------------------------------
@@ -327,36 +342,42 @@ __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* _
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  // entries of core rows, plus the rows of the borders this lane labels
-  // first (views without row offsets count the core rows only)
+  // entries of the border rows this lane labels first (views without row
+  // offsets count nothing); the core rows' entries were counted by k_core_mark
   uint32_t labeled = 0;
   for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
        base += waves * kWave * C) {
-    uint32_t row[C];
-    int32_t c[C];
+    // only similar entries can make a border: rows and neighbours are read
+    // for those alone.  The others get a marker row, so one row's lanes may
+    // form several runs; the segmented minimum may then also take lanes of
+    // the same row from another run, which are that row's candidates too,
+    // and each run's first lane applies it (an extra atomic at most).
     bool y[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
-      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
-      c[j] = k < m ? adj[k] : 0;
       y[j] = k < m && flag[k] == kFlagY;
     }
+    uint32_t row[C];
+    int32_t c[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
-      const bool rc = row[j] != 0xFFFFFFFFu && core[row[j]];
-      labeled += rc;
-      y[j] = y[j] && !rc && core[c[j]];
+      const uint64_t k = base + j * kWave + lane;
+      row[j] = y[j] ? src[k] : 0xFFFFFFFFu;
+      c[j] = y[j] ? adj[k] : 0;
     }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) y[j] = y[j] && core[c[j]] && !core[row[j]];
     int32_t cand[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) cand[j] = y[j] ? label[c[j]] : kNoLabel;
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       if (!__ballot(y[j])) continue;
-      const int32_t best = wave_seg_reduce(row[j], cand[j], OpMin());
-      if (wave_seg_head(row[j]) && best != kNoLabel && atomicMin(&label[row[j]], best) == kNoLabel && rb)
-        labeled += re[row[j]] - rb[row[j]];
+      const uint32_t r = y[j] ? row[j] : 0xFFFFFFFFu;
+      const int32_t best = wave_seg_reduce(r, cand[j], OpMin());
+      if (wave_seg_head(r) && best != kNoLabel && atomicMin(&label[r], best) == kNoLabel && rb)
+        labeled += re[r] - rb[r];
     }
   }
   block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
@@ -513,7 +534,8 @@ void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s)
 
 void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
                       const int32_t* mu_dev) {
-  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, p.mu, mu_dev, b.core, b.parent, b.label);
+  k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, p.mu, mu_dev, g->row_begin, g->row_end, b.core,
+                                                       b.parent, b.label, b.cnt);
 }
 
 void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
