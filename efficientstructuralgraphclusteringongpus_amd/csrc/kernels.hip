@@ -407,16 +407,21 @@ __global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, uint64_t m_all,
   for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
        base += waves * kWave * C) {
     uint32_t row[C];
-    int32_t nb[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
       row[j] = k < m ? src[k] : 0xFFFFFFFFu;
-      nb[j] = k < m ? adj[k] : 0;
     }
     int32_t lr[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) lr[j] = row[j] != 0xFFFFFFFFu ? label[row[j]] : kNoLabel;
+    // neighbours only where this side needs them
+    int32_t nb[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j)
+      nb[j] = row[j] != 0xFFFFFFFFu && (from_labeled ? lr[j] != kNoLabel : lr[j] == kNoLabel)
+                  ? adj[base + j * kWave + lane]
+                  : 0;
     if (from_labeled) {
       bool to[C];
 #pragma unroll
