@@ -232,6 +232,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_server_submit", C.c_int, [_VP, C.c_uint32, C.c_double, C.c_int32, _P(C.c_uint64)])
     _sig(lib, "sgc_server_progress", C.c_int, [_VP, C.c_int, _P(C.c_uint32)])
     _sig(lib, "sgc_server_pending", C.c_uint32, [_VP])
+    _sig(lib, "sgc_server_completed", C.c_int, [_VP, _VP, C.c_uint32, _P(C.c_uint32)])
     _sig(lib, "sgc_server_result", C.c_int, [_VP, C.c_uint64, _VP, C.c_uint64, _P(_QueryStats)])
     _sig(lib, "sgc_scan_ooc", C.c_int,
          [_VP, _VP, C.c_double, C.c_int32, C.c_uint64, _VP, _VP, _VP, _P(_Stats), _P(_OocStats)])
@@ -569,6 +570,13 @@ class Server:
     @property
     def pending(self) -> int:
         return int(self._lib.sgc_server_pending(self._h))
+
+    def completed(self, capacity: int = 4096) -> list[int]:
+        """Tickets retired since the last call (each listed once)."""
+        buf = np.empty(capacity, dtype=np.uint64)
+        count = C.c_uint32()
+        _check(self._lib.sgc_server_completed(self._h, _ptr(buf), int(capacity), C.byref(count)))
+        return [int(t) for t in buf[: count.value]]
 
     def result(self, ticket: int, n: int):
         """(labels, QueryStats) of a retired query, or None while it runs."""

@@ -44,23 +44,53 @@ struct QueryParams {
 // flag[k] for one query from cn[]: the degree-only rule first
 // (GPUSCAN++/GPUScan.cu:143-169), then for undecided edges the predicate
 // `(cn+2)*(cn+2) < eps*eps*(s+1)*(l+1)` -> not similar (GPUScan.cu:220-221).
-// eps comes from prm when given.
+// The same pass counts each row's similar entries into sd (the core count).
+// The neighbour degree comes from ndeg when the graph has it; eps comes from
+// prm when given.
 __global__ void __launch_bounds__(kBlock) k_threshold(uint32_t m, const uint32_t* __restrict__ rp,
                                                       const uint32_t* __restrict__ src,
-                                                      const int32_t* __restrict__ adj, const uint32_t* __restrict__ cn,
-                                                      double eps, const QueryParams* __restrict__ prm,
-                                                      uint8_t* __restrict__ flag) {
+                                                      const int32_t* __restrict__ adj,
+                                                      const uint32_t* __restrict__ ndeg,
+                                                      const uint32_t* __restrict__ cn, double eps,
+                                                      const QueryParams* __restrict__ prm,
+                                                      uint8_t* __restrict__ flag, uint32_t* __restrict__ sd) {
   if (prm) eps = prm->eps;
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride) {
-    const uint32_t u = src[k], v = (uint32_t)adj[k];
-    const uint32_t du = rp[u + 1] - rp[u], dv = rp[v + 1] - rp[v];
-    uint8_t f = prep_flag(u, v, du, dv, eps);
-    if (f >= kFlagUUp) {
-      const uint32_t s = f == kFlagUUp ? du : dv, l = f == kFlagUUp ? dv : du;
-      f = predicate_exact(cn[k], predicate_rhs(eps, s, l)) ? kFlagY : kFlagN;
+  constexpr uint32_t C = kEdgeChunks;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
+       base += waves * kWave * C) {
+    uint32_t u[C], v[C], du[C], dv[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
+      v[j] = k < m ? (uint32_t)adj[k] : 0u;
+      dv[j] = k < m && ndeg ? ndeg[k] : 0u;
     }
-    flag[k] = f;
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const bool in = u[j] != 0xFFFFFFFFu;
+      du[j] = in ? rp[u[j] + 1] - rp[u[j]] : 0u;
+      if (!ndeg) dv[j] = in ? rp[v[j] + 1] - rp[v[j]] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = base + j * kWave + lane;
+      uint32_t y = 0;
+      if (k < m) {
+        uint8_t f = prep_flag(u[j], v[j], du[j], dv[j], eps);
+        if (f >= kFlagUUp) {
+          const uint32_t s = f == kFlagUUp ? du[j] : dv[j], l = f == kFlagUUp ? dv[j] : du[j];
+          f = predicate_exact(cn[k], predicate_rhs(eps, s, l)) ? kFlagY : kFlagN;
+        }
+        flag[k] = f;
+        y = f == kFlagY;
+      }
+      if (!__ballot(y)) continue;
+      const uint32_t sum = wave_seg_reduce(u[j], y, OpAdd());
+      if (wave_seg_head(u[j]) && sum) atomicAdd(&sd[u[j]], sum);
+    }
   }
 }
 
@@ -139,6 +169,7 @@ struct sgc_server {
   std::unordered_map<uint32_t, sgc::Graph*> graphs;
   std::deque<sgc::Query> pending;
   std::unordered_map<uint64_t, sgc::Result> done;
+  std::deque<uint64_t> fresh;  // retired tickets not yet listed by sgc_server_completed
   std::unordered_map<uint64_t, uint32_t> inflight;  // ticket -> graph
   bool graph_launch = true;
 };
@@ -170,9 +201,8 @@ void enqueue_query(const Graph* G, const Slot& sl, const ScanParams* p, hipStrea
   const int32_t* mu_dev = p ? nullptr : &sl.d_params->mu;
   k_query_init<<<grid_for(g->n, kBlock), kBlock, 0, st>>>(g->n, sl.pb.sd, sl.pb.lo, sl.pb.hi, sl.pb.cnt);
   if (g->m)
-    k_threshold<<<grid_for(g->m, kBlock), kBlock, 0, st>>>(g->m, g->row_ptr, g->src, g->adj, G->cn, pv.eps, prm,
-                                                            sl.pb.flag);
-  launch_core_count(g, sl.pb, st);
+    k_threshold<<<grid_for(g->m, kBlock), kBlock, 0, st>>>(g->m, g->row_ptr, g->src, g->adj, g->ndeg, G->cn, pv.eps,
+                                                            prm, sl.pb.flag, sl.pb.sd);
   launch_core_mark(g, sl.pb, pv, st, mu_dev);
   launch_union(g, sl.pb, st);
   launch_flatten(g, sl.pb, st);
@@ -294,6 +324,7 @@ int retire(sgc_server* S, bool wait, int* retired) {
       r.st.hubs = c[kCntHubs];
       r.st.outliers = c[kCntOutliers];
       S->done.emplace(s.ticket, std::move(r));
+      S->fresh.push_back(s.ticket);
       S->inflight.erase(s.ticket);
       s.busy = false;
       ++*retired;
@@ -485,6 +516,19 @@ extern "C" int sgc_server_progress(sgc_server* S, int wait, uint32_t* completed)
   rc = dispatch(S);
   if (completed) *completed = (uint32_t)retired;
   return rc;
+}
+
+extern "C" int sgc_server_completed(sgc_server* S, uint64_t* tickets, uint32_t capacity, uint32_t* count) {
+  clear_error();
+  if (!S || !count || (capacity && !tickets)) return fail(SGC_EINVAL, "null argument");
+  uint32_t c = 0;
+  while (c < capacity && !S->fresh.empty()) {
+    const uint64_t t = S->fresh.front();
+    S->fresh.pop_front();
+    if (S->done.count(t)) tickets[c++] = t;  // not collected already
+  }
+  *count = c;
+  return SGC_OK;
 }
 
 extern "C" uint32_t sgc_server_pending(const sgc_server* S) {

@@ -307,6 +307,10 @@ int sgc_server_submit(sgc_server* srv, uint32_t graph_id, double eps, int32_t mu
 /* Retires finished queries and starts queued ones in the freed slots; with
  * wait, first blocks until the oldest running query finishes. */
 int sgc_server_progress(sgc_server* srv, int wait, uint32_t* completed);
+/* Up to `capacity` tickets retired since the last call (each listed once,
+ * unless its result was already taken), into tickets; *count of them.  Lets
+ * a caller with many open queries collect results without polling each. */
+int sgc_server_completed(sgc_server* srv, uint64_t* tickets, uint32_t capacity, uint32_t* count);
 /* Queries submitted and not yet retired. */
 uint32_t sgc_server_pending(const sgc_server* srv);
 /* A retired query's labels (n int32, as sgc_scan) and statistics; the result

@@ -155,3 +155,34 @@ def test_graph_replays_match_eager_launches(ctx, graphs):
             assert np.array_equal(results[(True, eps, mu)], results[(False, eps, mu)])
     finally:
         dg.close()
+
+
+def test_completed_lists_each_retired_ticket_once(ctx, graphs):
+    """sgc_server_completed lists every retired query exactly once, and the
+    results collected through it are the ones polling each ticket gives."""
+    say("completed tickets")
+    with sgc.Server(ctx, 1 << 32, slots_per_graph=2) as srv:
+        gids = [srv.add_graph(g)[0] for g in graphs]
+        ref = {}
+        tickets = {}
+        for k in range(24):
+            gi = k % len(graphs)
+            t = srv.submit(gids[gi], 0.2 + 0.1 * (k % 6), 2 + k % 5)
+            tickets[t] = (gi, 0.2 + 0.1 * (k % 6), 2 + k % 5)
+        seen = []
+        got = {}
+        while len(got) < len(tickets):
+            srv.progress(wait=True)
+            for t in srv.completed():
+                seen.append(t)
+                gi = tickets[t][0]
+                got[t] = srv.result(t, graphs[gi].n)
+        assert sorted(seen) == sorted(tickets) and len(seen) == len(set(seen))
+        assert srv.completed() == []
+        for t, (gi, eps, mu) in tickets.items():
+            dg = sgc.DeviceGraph(ctx, graphs[gi])
+            try:
+                one = sgc.scan(ctx, dg, eps, mu)
+            finally:
+                dg.close()
+            assert np.array_equal(got[t][0], one.labels)
