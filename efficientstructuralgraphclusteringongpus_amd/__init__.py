@@ -113,8 +113,8 @@ class _Tuning(C.Structure):
 @dataclass
 class Tuning:
     small_max_degree: int = 1024
-    small_unit_work: float = 65536.0
-    large_unit_work: float = 1048576.0
+    small_unit_work: float = 262144.0
+    large_unit_work: float = 4194304.0
     hash_tables: int = 1
     tiny_work: int = 512
     ms_similarity: float = 0.0

@@ -159,8 +159,8 @@ void sgc_dgraph_free(sgc_dgraph* dg);
  * depend on them, only the time. */
 typedef struct {
   uint32_t small_max_degree; /* owners of degree <= this take the small tier (1..1024; default 1024) */
-  double small_unit_work;    /* short-row elements per small-tier work unit (default 65536) */
-  double large_unit_work;    /* per workgroup-tier work unit (default 1048576) */
+  double small_unit_work;    /* short-row elements per small-tier work unit (default 262144) */
+  double large_unit_work;    /* per workgroup-tier work unit (default 4194304) */
   uint32_t hash_tables;      /* 1 (default): owner rows as cuckoo hashes where they fit (small tier; whole
                                 rows and runs of 2^20-id windows of <= 8192 keys in the workgroup tier);
                                 0: linear probing (small tier) and one bitmap window at a time only */
