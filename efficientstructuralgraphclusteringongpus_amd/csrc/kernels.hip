@@ -27,9 +27,13 @@ __global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t*
   }
 }
 
-// mirror[k] = position of the reverse entry (v, u) of entry k = (u, v), by
-// binary search of u in the sorted row of v (0xFFFFFFFF if the graph lacks
-// it); four searches per thread in lockstep so that their loads overlap.
+// mirror[k] = position of the reverse entry (v, u) of entry k = (u, v).  Of
+// the two entries of an edge only the one whose neighbour row is the shorter,
+// (d(v), v) <= (d(u), u), searches: a binary search of u in the sorted row of
+// v, which writes both mirror[k] and mirror[p].  A pair is found only from
+// entry k, so each word has one writer.  Entries whose reverse entry is
+// missing (asymmetric input) keep the 0xFFFFFFFF the caller fills in first.
+// Four searches per thread run in lockstep so that their loads overlap.
 __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* __restrict__ rp,
                                                    const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
                                                    uint32_t* __restrict__ mirror) {
@@ -38,16 +42,22 @@ __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* _
   for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
     uint32_t lo[C], hi[C], end[C];
     int32_t key[C];
+    bool on[C];
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = k0 + j * stride;
       lo[j] = hi[j] = end[j] = 0;
       key[j] = 0;
+      on[j] = false;
       if (k < m) {
-        const uint32_t v = (uint32_t)adj[k];
-        key[j] = (int32_t)src[k];
-        lo[j] = rp[v];
-        hi[j] = end[j] = rp[v + 1];
+        const uint32_t v = (uint32_t)adj[k], u = src[k];
+        const uint32_t vb = rp[v], ve = rp[v + 1], du = rp[u + 1] - rp[u], dv = ve - vb;
+        key[j] = (int32_t)u;
+        on[j] = dv < du || (dv == du && v <= u);
+        if (on[j]) {
+          lo[j] = vb;
+          hi[j] = end[j] = ve;
+        }
       }
     }
     for (;;) {
@@ -72,8 +82,13 @@ __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* _
         }
     }
 #pragma unroll
-    for (uint32_t j = 0; j < C; ++j)
-      if (k0 + j * stride < m) mirror[k0 + j * stride] = lo[j] < end[j] && adj[lo[j]] == key[j] ? lo[j] : 0xFFFFFFFFu;
+    for (uint32_t j = 0; j < C; ++j) {
+      if (!on[j]) continue;
+      const uint32_t k = (uint32_t)(k0 + j * stride);
+      const bool found = lo[j] < end[j] && adj[lo[j]] == key[j];
+      mirror[k] = found ? lo[j] : 0xFFFFFFFFu;
+      if (found) mirror[lo[j]] = k;
+    }
   }
 }
 
@@ -513,8 +528,10 @@ void launch_neighbor_degrees(const sgc_dgraph* g, hipStream_t s) {
 }
 
 void launch_mirror(const sgc_dgraph* g, hipStream_t s) {
-  if (g->m && g->mirror)
-    k_mirror<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->src, g->adj, g->mirror);
+  if (!g->m || !g->mirror) return;
+  // words of entries without a reverse entry are never written by k_mirror
+  (void)hipMemsetAsync(g->mirror, 0xFF, (size_t)g->m * 4, s);
+  k_mirror<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->src, g->adj, g->mirror);
 }
 
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
