@@ -564,9 +564,11 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
                 uint64_t* __restrict__ cnt, int units_slot, int queue_slot, bool hash_tables) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
-  __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges], e_s[T::kMaxEdges], e_cstar[T::kMaxEdges];
-  __shared__ uint32_t e_cur[T::kMaxEdges], e_hi[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges];
-  __shared__ uint32_t e_cap[T::kMaxEdges], e_a[T::kMaxEdges];
+  // (c*, d(a), claim cap, slice end) of each edge: what a wave needs to size
+  // a claim, in one 16-byte LDS read
+  __shared__ __align__(16) uint4 e_q[T::kMaxEdges];
+  __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges];
+  __shared__ uint32_t e_cur[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges], e_a[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend, s_fail;
   __shared__ uint4 s_next;  // the next unit's descriptor
   // the owner row's window boundaries, from the window-offset index
@@ -644,9 +646,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       e_k[t] = k;
       e_a[t] = a;
       e_ab[t] = ab;
-      e_s[t] = s;
+      e_q[t].y = s;
       e_cur[t] = ab;
-      e_cstar[t] = cstar;
+      e_q[t].x = cstar;
       e_mir[t] = k < m ? ed.y : kNone;
       e_state[t] = 0ull;
     }
@@ -752,9 +754,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
-        const uint32_t cur = e_cur[t], ab = e_ab[t], s = e_s[t];
+        const uint32_t cur = e_cur[t], ab = e_ab[t], s = e_q[t].y;
         e_pos[t] = cur;
-        e_hi[t] = ab + s;
+        e_q[t].w = ab + s;
         uint32_t cap = T::kChunk;
         if (exact) {
           // the edge's slice of this window; the elements skipped since the
@@ -767,7 +769,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
             if (lo > cur) e_state[t] += (unsigned long long)(lo - cur);
           }
           e_pos[t] = lo;
-          e_hi[t] = hi;
+          e_q[t].w = hi;
         } else if (!last) {
           // the short row's slice is taken to end where the owner row's
           // share of values below whi puts it
@@ -776,7 +778,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           const uint32_t c = (pred + pred / 4 + 2 * kWave - 1) / kWave * kWave;
           cap = c < T::kChunk ? c : T::kChunk;
         }
-        e_cap[t] = cap;
+        e_q[t].z = cap;
       }
       if (threadIdx.x == 0) s_cursor = 0;
       __syncthreads();  // table and per-edge cursors ready
@@ -786,14 +788,17 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       for (;;) {
         uint32_t e = kNone, start = 0, end = 0;
         if (cur_e != kNone) {
+          // one round of LDS reads, then the cursor atomic; a stale (higher) slice
+          // end only lets a claim start past a bitmap window's end, where
+          // it consumes nothing and finds no keys
           const unsigned long long st = e_state[cur_e];
-          const uint32_t cstar = e_cstar[cur_e], s_e = e_s[cur_e];
+          const uint4 q = e_q[cur_e];
+          const uint32_t cstar = q.x, s_e = q.y, cap = q.z, ehi = q.w;
           uint32_t got = 0xFFFFFFFFu;
           if (!edge_decided(st, cstar, s_e)) {
-            const uint32_t want = claim_size(st, cstar, s_e, e_cap[cur_e]);
+            const uint32_t want = claim_size(st, cstar, s_e, cap);
             if (lane == 0) got = atomicAdd(&e_pos[cur_e], want);
-            got = __shfl(got, 0);
-            const uint32_t ehi = e_hi[cur_e];
+            got = __builtin_amdgcn_readfirstlane(got);  // lane 0's, without an LDS permute
             if (got < ehi) {
               e = cur_e;
               start = got;
@@ -805,7 +810,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         if (e == kNone) {
           uint32_t t = 0;
           if (lane == 0) t = atomicAdd(&s_cursor, 1u);
-          t = __shfl(t, 0);
+          t = __builtin_amdgcn_readfirstlane(t);
           if (t < ne) {
             cur_e = t;
             continue;
@@ -814,7 +819,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           uint32_t found = kNone;
           for (uint32_t base = 0; base < ne && found == kNone; base += kWave) {
             const uint32_t q = base + lane;
-            const bool open = q < ne && e_pos[q] < e_hi[q] && !edge_decided(e_state[q], e_cstar[q], e_s[q]);
+            const bool open = q < ne && e_pos[q] < e_q[q].w && !edge_decided(e_state[q], e_q[q].x, e_q[q].y);
             const uint64_t bm = __ballot(open);
             if (bm) found = base + (uint32_t)(__ffsll((long long)bm) - 1);
           }
@@ -830,9 +835,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         const uint32_t nl = (len + kWave - 1) / kWave;
         uint32_t h;
         if (hashed)
-          h = count_dispatch<T, true>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
+          h = count_dispatch<T, true>(nl, cc, s0, len, &e_mir[e], &e_q[e].w, &used);
         else
-          h = count_dispatch<T, false>(nl, cc, s0, len, &e_mir[e], &e_hi[e], &used);
+          h = count_dispatch<T, false>(nl, cc, s0, len, &e_mir[e], &e_q[e].w, &used);
         if (lane == 0) atomicAdd(&e_state[e], ((unsigned long long)h << 32) | (unsigned long long)used);
         streamed += used;
         loaded += len;
@@ -844,7 +849,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       SGC_TICK(2);
       __syncthreads();
       SGC_TICK(3);
-      if (threadIdx.x < ne) e_cur[threadIdx.x] = e_hi[threadIdx.x];
+      if (threadIdx.x < ne) e_cur[threadIdx.x] = e_q[threadIdx.x].w;
       kcur = kend;
       __syncthreads();
     }
@@ -854,8 +859,8 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       const uint32_t k = e_k[t];
       if (k < m) {
         const uint32_t hits = (uint32_t)(e_state[t] >> 32);
-        const uint32_t cstar = e_cstar[t];
-        const uint32_t ab = e_ab[t], s = e_s[t];
+        const uint32_t cstar = e_q[t].x;
+        const uint32_t ab = e_ab[t], s = e_q[t].y;
         const bool sim = cstar != kNone ? hits >= cstar : predicate_exact(hits, predicate_rhs(eps, s, l));
         uint32_t mpos = e_mir[t];
         if (mpos == kNone && !mirror) {
