@@ -669,13 +669,22 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       // ---- LDS table for owner keys adj[kcur, kend) of the ids [wlo, whi)
       // (windows w0 .. w0 + wspan - 1).  Small tier: hash table of the whole row.
       uint32_t kend = be, wlo = 0, whi = 0xFFFFFFFFu, mask = 0, shift = 0, wspan = 1;
+      uint32_t pre_lo = 0, pre_hi = 0;  // this thread's edge slice (exact passes)
       bool hashed = false;
       if constexpr (T::kBitmap) {
         uint32_t w0 = 0, w1 = 0;
         if (hash_ok && whole) {
           hashed = true;
         } else {
-          w0 = (uint32_t)adj[kcur] >> kWindowBits;
+          if (bw) {
+            // the window of the first key left, from the owner's window
+            // boundaries in LDS: the last w with s_bw[w] <= kcur (s_bw[0] is
+            // the row start, so one lane always qualifies)
+            const uint64_t le = __ballot(lane < nwin && s_bw[lane] <= kcur);
+            w0 = 63u - (uint32_t)__clzll((long long)le);
+          } else {
+            w0 = (uint32_t)adj[kcur] >> kWindowBits;
+          }
           w1 = w0 + 1;
           if (hash_ok && nwin > 1)
             while (w1 < nwin && s_bw[w1 + 1] - kcur <= kCuckooMaxKeys) ++w1;
@@ -683,6 +692,13 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           wlo = w0 << kWindowBits;
           whi = w1 < nwin || !bw ? wlo + ((w1 - w0) << kWindowBits) : 0xFFFFFFFFu;
           wspan = w1 - w0;
+        }
+        // the edges' slices of this pass from the window-offset index, loaded
+        // before the table is built so that their latency overlaps it
+        if (woff && !(hashed && whole) && threadIdx.x < ne && e_k[threadIdx.x] < m) {
+          const uint32_t* wo = woff + (size_t)e_a[threadIdx.x] * wstride + (wlo >> kWindowBits);
+          pre_lo = wo[0];
+          pre_hi = wo[wspan];
         }
         if (hashed) {
           if (!whole) kend = s_bw[w1];
@@ -763,9 +779,8 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           // last window lie in windows without owner keys: consumed, no hits
           uint32_t lo = cur, hi = cur;
           if (e_k[t] < m) {
-            const uint32_t* wo = woff + (size_t)e_a[t] * wstride + (wlo >> kWindowBits);
-            lo = wo[0];
-            hi = wo[wspan];
+            lo = pre_lo;
+            hi = pre_hi;
             if (lo > cur) e_state[t] += (unsigned long long)(lo - cur);
           }
           e_pos[t] = lo;
