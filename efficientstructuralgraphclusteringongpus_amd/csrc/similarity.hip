@@ -737,7 +737,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         const uint32_t slots = table_size_for(l);
         mask = slots;
         shift = 32 - log2_pow2(slots);
-        for (uint32_t t = threadIdx.x; t < 2 * slots; t += T::kThreads) tab[t] = kEmptySlot;
+        uint4* tab4 = reinterpret_cast<uint4*>(tab);  // 2 * slots is a multiple of 128
+        for (uint32_t t = threadIdx.x; t < 2 * slots / 4; t += T::kThreads)
+          tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
         if (threadIdx.x == 0) s_fail = 0;
         __syncthreads();
         bool ok = true;
@@ -755,7 +757,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         const uint32_t TS = table_size_for(2 * l);
         mask = TS - 1;
         shift = 32 - log2_pow2(TS);
-        for (uint32_t t = threadIdx.x; t < TS; t += T::kThreads) tab[t] = kEmptySlot;
+        uint4* tab4 = reinterpret_cast<uint4*>(tab);  // TS is a multiple of 64
+        for (uint32_t t = threadIdx.x; t < TS / 4; t += T::kThreads)
+          tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < KPT; ++j)
