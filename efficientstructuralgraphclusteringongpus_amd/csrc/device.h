@@ -39,6 +39,7 @@ enum Counter {
   kCntClaims,   // claims taken
   kCntDense,    // owned undecided entries listed for the one-wave-per-edge kernel
   kCntLabeled,  // entries of labelled rows (cores, borders): picks the classify pass's side
+  kCntTiny,     // tiny owners listed by the planner for the one-thread-per-owner kernel
   kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32
 };
@@ -131,11 +132,14 @@ struct PassBuffers {
   uint32_t* cnt_u;   // n: undecided edges owned by each vertex (kFlagUDown entries)
   uint64_t* work_u;  // n: sum of the short-row lengths of those edges
   // m: owner b's undecided edges at [rb[b], rb[b] + cnt_u[b]) as (entry,
-  // a, row start of a, d(a))
+  // its reverse entry, row start of a, d(a))
   uint4* uedge;
-  // ucap work units (begin, end in uedge, owner, owner's row start): small
-  // tier from the front, workgroup tier from the back
+  // ucap work units (begin, end in uedge, the owner's row begin and end):
+  // small tier from the front, workgroup tier from the back
   uint4* units;
+  // n: ids of the tiny owners (cnt[kCntTiny] of them).  Shares label's
+  // memory, which is free until the core pass writes it.
+  uint32_t* tiny;
   uint32_t ucap;
   uint64_t* cnt;     // kCntSlots
 };

@@ -63,6 +63,7 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
   pb->work_u = (uint64_t*)(base + o_work_u);
   pb->uedge = (uint4*)(base + o_uedge);
   pb->units = (uint4*)(base + o_units);
+  pb->tiny = (uint32_t*)pb->label;
   pb->ucap = (uint32_t)unit_capacity(m);
   pb->cnt = (uint64_t*)ctx->counters.ptr;
   return SGC_OK;

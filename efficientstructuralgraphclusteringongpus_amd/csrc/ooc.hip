@@ -222,6 +222,7 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   pb.sd = (uint32_t*)(base + RLay.sd);
   pb.parent = (int32_t*)(base + RLay.par);
   pb.label = (int32_t*)(base + RLay.lab);
+  pb.tiny = (uint32_t*)pb.label;
   pb.lo = (int32_t*)(base + RLay.lo);
   pb.hi = (int32_t*)(base + RLay.hi);
   pb.core = (uint8_t*)(base + RLay.core);

@@ -185,22 +185,23 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
                                                  uint4* __restrict__ units, uint32_t ucap,
-                                                 uint64_t* __restrict__ cnt) {
+                                                 uint32_t* __restrict__ tiny, uint64_t* __restrict__ cnt) {
   constexpr uint32_t V = 4;  // vertices per thread per block iteration
-  unsigned long long* const ctr[2] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2]};
+  unsigned long long* const ctr[3] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2],
+                                      (unsigned long long*)&cnt[kCntTiny]};
   const uint64_t tile = (uint64_t)kBlock * V;
   for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {  // block-uniform
-    uint32_t owned[V], c1[V], c2[V], per[V], start[V], stop[V];
+    uint32_t owned[V], c1[V], c2[V], c3[V], per[V], start[V], stop[V];
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint64_t x = base + j * kBlock + threadIdx.x;
       owned[j] = x < n ? cnt_u[x] : 0u;
     }
-    uint32_t need[2] = {0, 0};
+    uint32_t need[3] = {0, 0, 0};
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint64_t x = base + j * kBlock + threadIdx.x;
-      c1[j] = c2[j] = 0;
+      c1[j] = c2[j] = c3[j] = 0;
       per[j] = 1;
       start[j] = stop[j] = 0;
       if (owned[j]) {
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
         stop[j] = re[x];
         const uint32_t d = stop[j] - start[j];
         if (tiny_owner(owned[j], work_u[x], d, tune.tiny_work)) {
-          // decided by k_sim_tiny
+          c3[j] = 1;  // decided by k_sim_tiny, from the tiny-owner list
         } else if (d <= tune.small_max_degree) {
           per[j] = edges_per_unit(owned[j], work_u[x], tune.small_unit_work, kT1MaxEdges);
           c1[j] = (owned[j] + per[j] - 1) / per[j];
@@ -219,8 +220,9 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       }
       need[0] += c1[j];
       need[1] += c2[j];
+      need[2] += c3[j];
     }
-    block_reserve<2>(need, ctr);
+    block_reserve<3>(need, ctr);
     // small-tier units from the front of `units`, workgroup-tier units from
     // the back: (begin, end) in uedge, the owner's row [begin, end) in adj
 #pragma unroll
@@ -233,28 +235,31 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
         const uint32_t at = c1[j] ? need[0] + t : ucap - 1 - (need[1] + t);
         units[at] = make_uint4((uint32_t)j0, (uint32_t)j1, start[j], stop[j]);
       }
+      if (c3[j]) tiny[need[2]] = (uint32_t)(base + j * kBlock + threadIdx.x);
       need[0] += c1[j];
       need[1] += c2[j];
+      need[2] += c3[j];
     }
   }
 }
 
-// Tiny owners (tiny_owner): one thread per owner merges the short row and the
-// owner row of each of its undecided edges in turn, with the same exact
-// threshold and early exit as the tiered kernels, and writes both entries.
-__global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t n, uint32_t m, const uint32_t* __restrict__ rb,
+// Tiny owners (tiny_owner), listed by the planner: one thread per owner
+// merges the short row and the owner row of each of its undecided edges in
+// turn, with the same exact threshold and early exit as the tiered kernels,
+// and writes both entries.  The list keeps a wave's lanes on owners with
+// work, instead of one lane in twelve on a grid over all vertices.
+__global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t* __restrict__ rb,
                                                      const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ mirror, double eps,
-                                                     uint32_t tiny_work, const uint32_t* __restrict__ cnt_u,
-                                                     const uint64_t* __restrict__ work_u,
+                                                     const uint32_t* __restrict__ cnt_u,
+                                                     const uint32_t* __restrict__ tiny,
                                                      const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
-                                                     uint32_t* __restrict__ cn_out) {
+                                                     uint32_t* __restrict__ cn_out, const uint64_t* __restrict__ cnt) {
+  const uint64_t count = cnt[kCntTiny];
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t xi = (uint64_t)blockIdx.x * kBlock + threadIdx.x; xi < n; xi += stride) {
-    const uint32_t x = (uint32_t)xi, owned = cnt_u[x];
-    if (!owned) continue;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+    const uint32_t x = tiny[i], owned = cnt_u[x];
     const uint32_t bb = rb[x], be = re[x], l = be - bb;
-    if (!tiny_owner(owned, work_u[x], l, tiny_work)) continue;
     for (uint32_t i = 0; i < owned; ++i) {
       const uint4 ed = uedge[bb + i];  // (entry, reverse entry, row start of a, d(a))
       const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
@@ -971,7 +976,7 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
   k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
-                                                   b.ucap, b.cnt);
+                                                   b.ucap, b.tiny, b.cnt);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
@@ -991,10 +996,14 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
                                                                  b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
                                                                  b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0);
-  if (g->tuning.tiny_work)
-    k_sim_tiny<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->m, g->row_begin, g->row_end, g->adj, g->mirror, p.eps,
-                                                         g->tuning.tiny_work, b.cnt_u, b.work_u, b.uedge, b.flag,
-                                                         cn_out);
+  if (g->tuning.tiny_work) {
+    // the list's length is on the device: enough threads for every owner of
+    // the chip's vertices, grid-striding beyond two per CU slot
+    const uint64_t want = ((uint64_t)g->n + kBlock - 1) / kBlock, cap = (uint64_t)cus * 32;
+    k_sim_tiny<<<(uint32_t)(want < cap ? want : cap), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj,
+                                                                     g->mirror, p.eps, b.cnt_u, b.tiny, b.uedge,
+                                                                     b.flag, cn_out, b.cnt);
+  }
 }
 
 }  // namespace sgc
