@@ -202,6 +202,8 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_graph_max_degree", C.c_uint32, [_VP])
     _sig(lib, "sgc_graph_digest", C.c_uint64, [_VP])
     _sig(lib, "sgc_graph_free", None, [_VP])
+    _sig(lib, "sgc_graph_pin", C.c_int, [_VP])
+    _sig(lib, "sgc_graph_pinned", C.c_int, [_VP])
     _sig(lib, "sgc_graph_similarity_work", C.c_int,
          [_VP, C.c_double, _P(C.c_uint64), _P(C.c_double), _P(C.c_double), _P(C.c_double)])
     _sig(lib, "sgc_philox4x32_10", None, [_VP, _VP, _VP])
@@ -351,6 +353,15 @@ class HostGraph:
     @property
     def max_degree(self) -> int:
         return int(self._lib.sgc_graph_max_degree(self._h))
+
+    def pin(self) -> None:
+        """Moves the adjacency into page-locked host memory (needs a device);
+        scan_ooc then sends row blocks without a staging copy."""
+        _check(self._lib.sgc_graph_pin(self._h))
+
+    @property
+    def pinned(self) -> bool:
+        return bool(self._lib.sgc_graph_pinned(self._h))
 
     def degrees(self) -> np.ndarray:
         return np.diff(self.row_ptr).astype(np.int64)

@@ -605,6 +605,28 @@ extern "C" const uint64_t* sgc_graph_row_ptr(const sgc_host_graph* g) { return g
 extern "C" const int32_t* sgc_graph_adj(const sgc_host_graph* g) { return g ? g->adj : nullptr; }
 extern "C" void sgc_graph_free(sgc_host_graph* g) { free_graph(g); }
 
+extern "C" int sgc_graph_pin(sgc_host_graph* g) {
+  clear_error();
+  if (!g) return fail(SGC_EINVAL, "null graph");
+  if (g->pinned) return SGC_OK;
+  int32_t* p = (int32_t*)host_alloc(g->m * sizeof(int32_t), true);
+  if (!p) {
+    (void)hipGetLastError();
+    return fail(SGC_EDEVICE, "page-locked host allocation of %llu bytes failed",
+                (unsigned long long)(g->m * sizeof(int32_t)));
+  }
+  const uint64_t piece = 1 << 20;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (uint64_t q = 0; q < g->m; q += piece)
+    memcpy(p + q, g->adj + q, (size_t)std::min(piece, g->m - q) * sizeof(int32_t));
+  host_free(g->adj, false);
+  g->adj = p;
+  g->pinned = true;
+  return SGC_OK;
+}
+
+extern "C" int sgc_graph_pinned(const sgc_host_graph* g) { return g && g->pinned ? 1 : 0; }
+
 extern "C" uint32_t sgc_graph_max_degree(const sgc_host_graph* g) {
   if (!g) return 0;
   uint64_t best = 0;

@@ -269,14 +269,16 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     if (sg.pending) SGC_HIP(hipEventSynchronize(sg.free));
     sg.pending = false;
     const double tb = now_ms();
-    ooc_copy_rows(g, blocks[b], blocks[b + 1], base_off, sg.adj, sg.off);
+    // a page-locked graph is sent as it lies; only the offsets are staged
+    ooc_copy_rows(g, blocks[b], blocks[b + 1], base_off, g->pinned ? nullptr : sg.adj, sg.off);
     R.ms_host += now_ms() - tb;
     return SGC_OK;
   };
   auto upload_block = [&](Staging& sg, size_t b, int32_t* dst, uint32_t* off_dev) -> int {
     const uint64_t m_b = g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]];
     const uint32_t nr = (uint32_t)(blocks[b + 1] - blocks[b]);
-    if (m_b) SGC_HIP(hipMemcpyAsync(dst, sg.adj, m_b * 4, hipMemcpyHostToDevice, s));
+    const int32_t* from = g->pinned ? g->adj + g->row_ptr[blocks[b]] : sg.adj;
+    if (m_b) SGC_HIP(hipMemcpyAsync(dst, from, m_b * 4, hipMemcpyHostToDevice, s));
     SGC_HIP(hipMemcpyAsync(off_dev, sg.off, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
     SGC_HIP(hipEventRecord(sg.free, s));
     sg.pending = true;

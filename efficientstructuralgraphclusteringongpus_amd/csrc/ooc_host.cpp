@@ -46,6 +46,7 @@ void ooc_copy_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, uint32_t b
   const uint64_t nr = r1 - r0;
 #pragma omp parallel for schedule(static)
   for (uint64_t i = 0; i <= nr; ++i) off[i] = base + (uint32_t)(rp[r0 + i] - b);
+  if (!dst) return;
   const uint64_t piece = 1 << 20;
 #pragma omp parallel for schedule(dynamic, 1)
   for (uint64_t p = 0; p < m; p += piece)
@@ -54,9 +55,13 @@ void ooc_copy_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, uint32_t b
 
 void ooc_merge_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, const uint8_t* chip_flag, uint8_t* flag) {
   const uint64_t b = g->row_ptr[r0], m = g->row_ptr[r1] - b;
+  uint8_t* out = flag + b;
+  // a blend rather than a conditional store, so that the loop vectorises
 #pragma omp parallel for schedule(static)
-  for (uint64_t k = 0; k < m; ++k)
-    if (chip_flag[k] <= kFlagY) flag[b + k] = chip_flag[k];
+  for (uint64_t k = 0; k < m; ++k) {
+    const uint8_t c = chip_flag[k], o = out[k];
+    out[k] = c <= kFlagY ? c : o;
+  }
 }
 
 }  // namespace sgc

@@ -19,8 +19,8 @@ void ooc_degrees(const sgc_host_graph* g, uint32_t* deg);
 // SGC_ENOMEM if one row alone exceeds cap.
 int ooc_plan_rows(const sgc_host_graph* g, uint64_t cap, std::vector<uint64_t>* bounds, uint64_t* max_entries);
 
-// Copies the entries of rows [r0, r1) to dst (parallel) and writes their
-// offsets, shifted by `base`, to off[0 .. r1-r0].
+// Copies the entries of rows [r0, r1) to dst (parallel; skipped for a null
+// dst) and writes their offsets, shifted by `base`, to off[0 .. r1-r0].
 void ooc_copy_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, uint32_t base, int32_t* dst, uint32_t* off);
 
 // Copies the final flags (N or Y) of rows [r0, r1), held in chip_flag, into

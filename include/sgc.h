@@ -104,6 +104,14 @@ uint32_t sgc_graph_max_degree(const sgc_host_graph* g);
 /* FNV-1a 64 over n, m, row_ptr and adj (little-endian words). */
 uint64_t sgc_graph_digest(const sgc_host_graph* g);
 void sgc_graph_free(sgc_host_graph* g);
+/* Moves g's adjacency into page-locked host memory (hipHostMalloc), so that
+ * sgc_scan_ooc sends row blocks to the device straight from the graph
+ * instead of through a staging copy.  No counterpart in the reference, whose
+ * out-of-core tier copies from pageable memory (GPUSCAN++O/SCANoffload.cu:
+ * 922-1464).  SGC_EDEVICE without a device or when the allocation fails; g
+ * is unchanged then.  Calling it on a pinned graph does nothing. */
+int sgc_graph_pin(sgc_host_graph* g);
+int sgc_graph_pinned(const sgc_host_graph* g);
 
 /* Replaces getEidAndEdgeList (GPUSCAN++/Graph.cpp:52-141): orients every
  * undirected edge from its lower (degree, id) endpoint to the other one.

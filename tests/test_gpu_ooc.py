@@ -58,6 +58,25 @@ def test_ooc_matches_resident_rmat(ctx, scale, ef, eps, mu, parts):
     assert_same(r, ref)
 
 
+def test_ooc_from_a_pinned_graph(ctx):
+    """Row blocks sent straight from a page-locked graph give the results of
+    the staged path, over many similarity chips and row chips."""
+    say("ooc pinned")
+    g = sgc.HostGraph.rmat(16, 16, seed=7)
+    digest = g.digest()
+    ref = resident(ctx, g, 0.4, 3)
+    budget = budget_for(g, 0.4, g.m // 10)
+    staged, info0 = sgc.scan_ooc(ctx, g, 0.4, 3, budget, want_sim=True)
+    g.pin()
+    assert g.pinned and g.digest() == digest
+    g.pin()  # a second call does nothing
+    pinned, info1 = sgc.scan_ooc(ctx, g, 0.4, 3, budget, want_sim=True)
+    say("chips", info1.similarity_chips, info1.row_chips)
+    assert info1.similarity_chips == info0.similarity_chips > 1
+    assert_same(staged, ref)
+    assert_same(pinned, ref)
+
+
 @pytest.mark.parametrize("eps", [0.8, 0.95])
 def test_ooc_row_chips(ctx, eps):
     """Few undecided edges keep the minimum budget small, so the passes after

@@ -121,3 +121,19 @@ def test_error_paths_do_not_exit():
         sgc.HostGraph.from_csr([0, 1], [0])  # self loop
     with pytest.raises(sgc.SGCError):
         sgc.HostGraph.from_edges(2, [0], [5])
+
+
+def test_pin_fails_cleanly_or_keeps_the_graph():
+    """Moving the adjacency into page-locked memory needs a device: without
+    one the call reports SGC_EDEVICE and the graph stays usable; with one the
+    graph keeps its bytes."""
+    g = sgc.HostGraph.rmat(10, 8, seed=3)
+    digest = g.digest()
+    try:
+        g.pin()
+    except sgc.SGCError as e:
+        assert e.code == 5
+        assert not g.pinned
+    else:
+        assert g.pinned
+    assert g.digest() == digest
