@@ -454,8 +454,9 @@ __device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x, uint32_t shift) {
   return ((x ^ (x >> 16)) * 0x85EBCA77u) >> shift;
 }
 
-__device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key, uint32_t slots, uint32_t shift) {
-  uint32_t k = key, side = 0;
+__device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key, uint32_t slots, uint32_t shift,
+                                              uint32_t side = 0) {
+  uint32_t k = key;
   for (int i = 0; i < kCuckooKicks; ++i) {
     const uint32_t slot = side ? slots + cuckoo_h1(k, shift) : cuckoo_h0(k, shift);
     const uint32_t old = atomicExch(&tab[slot], k);
@@ -464,6 +465,24 @@ __device__ __forceinline__ bool cuckoo_insert(uint32_t* tab, uint32_t key, uint3
     side ^= 1u;
   }
   return false;
+}
+
+// Inserts this thread's N keys (kEmptySlot: none): every first placement
+// into table 0 is issued before any result is looked at, so a thread's keys
+// cost one LDS round trip when nothing is evicted (the common case at these
+// loads); an evicted key continues its chain from table 1.
+template <uint32_t N>
+__device__ __forceinline__ bool cuckoo_insert_all(uint32_t* tab, const uint32_t (&keys)[N], uint32_t slots,
+                                                  uint32_t shift) {
+  uint32_t ev[N];
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j)
+    ev[j] = keys[j] != kEmptySlot ? atomicExch(&tab[cuckoo_h0(keys[j], shift)], keys[j]) : kEmptySlot;
+  bool ok = true;
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j)
+    if (ev[j] != kEmptySlot) ok = cuckoo_insert(tab, ev[j], slots, shift, 1u) && ok;
+  return ok;
 }
 
 // One claim of NL = ceil(len / 64) wave loads, straight-line so that all its
@@ -707,16 +726,23 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         }
         if (hashed) {
           if (!whole) kend = s_bw[w1];
+          // at most kCuckooMaxKeys keys: all of a thread's keys are loaded
+          // before the table is cleared, then inserted together
+          constexpr uint32_t KB = (kCuckooMaxKeys + T::kThreads - 1) / T::kThreads;
+          uint32_t kk[KB];
+#pragma unroll
+          for (uint32_t j = 0; j < KB; ++j) {
+            const uint32_t t = kcur + threadIdx.x + j * T::kThreads;
+            kk[j] = t < kend ? (uint32_t)adj[t] : kEmptySlot;
+          }
           uint4* tab4 = reinterpret_cast<uint4*>(tab);
           for (uint32_t t = threadIdx.x; t < 2 * kCuckooSlots / 4; t += T::kThreads)
             tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
           if (threadIdx.x == 0) s_fail = 0;
           __syncthreads();
-          bool ok = true;
           mask = kCuckooSlots;
           shift = kCuckooShift;
-          for (uint32_t t = kcur + threadIdx.x; t < kend; t += T::kThreads)
-            ok = cuckoo_insert(tab, (uint32_t)adj[t], kCuckooSlots, kCuckooShift) && ok;
+          const bool ok = cuckoo_insert_all<KB>(tab, kk, kCuckooSlots, kCuckooShift);
           if (!ok) s_fail = 1;
           __syncthreads();
           if (s_fail) {  // redo this pass with bitmap windows
@@ -747,7 +773,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           tab4[t] = make_uint4(kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot);
         if (threadIdx.x == 0) s_fail = 0;
         __syncthreads();
-        bool ok = true;
+        bool ok = true;  // key by key: batched placements cost this tier register spills
 #pragma unroll
         for (uint32_t j = 0; j < KPT; ++j)
           if (threadIdx.x + j * T::kThreads < l) ok = cuckoo_insert(tab, key[j], slots, shift) && ok;
