@@ -40,6 +40,7 @@ enum Counter {
   kCntDense,    // owned undecided entries listed for the one-wave-per-edge kernel
   kCntLabeled,  // entries of labelled rows (cores, borders): picks the classify pass's side
   kCntTiny,     // tiny owners listed by the planner for the one-thread-per-owner kernel
+  kCntOverflow, // owned undecided entries, when more than the unit array holds (asymmetric input)
   kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32
 };
@@ -155,6 +156,11 @@ int carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb);
 // Units of both tiers together are bounded by the number of owned undecided
 // edges: every edge is owned once and has two entries, so at most m / 2.
 inline size_t unit_capacity(uint64_t m) { return (size_t)(m / 2) + 2; }
+
+// Status for a pass whose planner found more owned undecided entries than
+// unit_capacity (counters as read back from the device): an input in which
+// some entry has no reverse entry.  SGC_OK otherwise.
+int check_unit_overflow(const uint64_t* cnt_host, uint64_t m);
 
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.

@@ -260,6 +260,16 @@ extern "C" int sgc_dgraph_set_tuning(sgc_dgraph* d, const sgc_tuning* t) {
 extern "C" uint64_t sgc_dgraph_n(const sgc_dgraph* d) { return d ? d->n : 0; }
 extern "C" uint64_t sgc_dgraph_m(const sgc_dgraph* d) { return d ? d->m : 0; }
 
+namespace sgc {
+int check_unit_overflow(const uint64_t* c, uint64_t m) {
+  if (!c[kCntOverflow]) return SGC_OK;
+  return fail(SGC_EFORMAT,
+              "the graph is not symmetric: %llu undecided entries are owned, more than the %llu work units a "
+              "graph of %llu entries with every reverse entry can need",
+              (unsigned long long)c[kCntOverflow], (unsigned long long)unit_capacity(m), (unsigned long long)m);
+}
+}  // namespace sgc
+
 namespace {
 
 int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* pb, sgc_stats* st) {
@@ -286,6 +296,7 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
   SGC_HIP(hipGetLastError());
   SGC_HIP(hipMemcpyAsync(ctx->host_counters, pb->cnt, kCntSlots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   SGC_HIP(hipStreamSynchronize(s));
+  if (int rc = check_unit_overflow(ctx->host_counters, g->m)) return rc;
   if (st) {
     memset(st, 0, sizeof(*st));
     st->ms_prep = elapsed(ctx->ev[0], ctx->ev[1]);
@@ -338,7 +349,9 @@ int time_similarity(sgc_ctx* ctx, sgc_dgraph* g, double eps, PassBuffers* pb, do
   launch_similarity(g, *pb, p, ctx->cu_count, kSimTiered, nullptr, s);
   SGC_HIP(hipEventRecord(ctx->ev[2], s));
   SGC_HIP(hipGetLastError());
+  SGC_HIP(hipMemcpyAsync(ctx->host_counters, pb->cnt, kCntSlots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   SGC_HIP(hipStreamSynchronize(s));
+  if (int rc = check_unit_overflow(ctx->host_counters, g->m)) return rc;
   *ms = elapsed(ctx->ev[1], ctx->ev[2]);
   return SGC_OK;
 }

@@ -361,6 +361,7 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     }
   }
   if (chip_no && (rc = merge(prev, chip_no - 1))) return rc;
+  if ((rc = check_unit_overflow(totals, sim_entries))) return rc;
   const size_t nsim = chip_no;
   progress("ooc: similarity done over %zu chips of %zu blocks", nsim, P);
 

@@ -424,6 +424,16 @@ extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uin
       free_graph(G);
       return fail(SGC_EDEVICE, "common-neighbour pass failed: %s", hipGetErrorString(e));
     }
+    uint64_t c[kCntSlots];
+    if (hipMemcpy(c, pb.cnt, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      free_graph(G);
+      return fail(SGC_EDEVICE, "reading the common-neighbour pass counters failed");
+    }
+    if ((rc = check_unit_overflow(c, m))) {
+      free_graph(G);
+      return rc;
+    }
   }
   G->precompute_ms = now_ms() - t0;
   // query slots

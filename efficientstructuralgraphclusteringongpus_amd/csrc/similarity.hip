@@ -187,6 +187,15 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                                  uint4* __restrict__ units, uint32_t ucap,
                                                  uint32_t* __restrict__ tiny, uint64_t* __restrict__ cnt) {
   constexpr uint32_t V = 4;  // vertices per thread per block iteration
+  // Every unit holds at least one owned entry, so ucap units always suffice
+  // when each undecided edge is owned once.  An input without a reverse
+  // entry for every entry can own more: then no unit is written (the
+  // regions from both ends would meet), and the host reports the count.
+  const uint64_t owned_all = cnt[kCntUEdges];
+  if (owned_all > ucap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCntOverflow] = owned_all;
+    return;
+  }
   unsigned long long* const ctr[3] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2],
                                       (unsigned long long*)&cnt[kCntTiny]};
   const uint64_t tile = (uint64_t)kBlock * V;
