@@ -252,11 +252,78 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
   }
 }
 
-// Tiny owners (tiny_owner), listed by the planner: one thread per owner
-// merges the short row and the owner row of each of its undecided edges in
-// turn, with the same exact threshold and early exit as the tiered kernels,
-// and writes both entries.  The list keeps a wave's lanes on owners with
-// work, instead of one lane in twelve on a grid over all vertices.
+// One undecided edge of a tiny owner x (row [bb, be)): merges the short row
+// and the owner row with the same exact threshold and early exit as the
+// tiered kernels, and writes both entries.
+__device__ __forceinline__ void tiny_edge(uint32_t m, uint32_t x, uint32_t bb, uint32_t be, uint32_t i,
+                                          const int32_t* __restrict__ adj, const uint32_t* __restrict__ mirror,
+                                          double eps, const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
+                                          uint32_t* __restrict__ cn_out) {
+  const uint32_t l = be - bb;
+  const uint4 ed = uedge[bb + i];  // (entry, reverse entry, row start of a, d(a))
+  const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
+  if (k >= m) return;
+  const double rhs = predicate_rhs(eps, s, l);
+  const uint32_t cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
+  // merge in blocks of 4 entries per row: count the matches between the
+  // two current blocks, then advance the block with the smaller last
+  // value (both if equal).  A matching pair is current together exactly
+  // once, and cn + min(entries left in either row) bounds the final
+  // count, so the early exit is the tiered kernels'.
+  uint32_t ia = ab, ib = bb, cn = 0, mpos = kNone;
+  while (ia < ae && ib < be) {
+    if (cstar != kNone) {
+      const uint32_t rest = ae - ia < be - ib ? ae - ia : be - ib;
+      if (cn >= cstar || cn + rest < cstar) break;
+    }
+    const uint32_t na = ae - ia < 4 ? ae - ia : 4, nb = be - ib < 4 ? be - ib : 4;
+    int32_t va[4], vb[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      va[q] = q < na ? adj[ia + q] : 0;
+      vb[q] = q < nb ? adj[ib + q] : 0;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      if (q < na && va[q] == (int32_t)x) mpos = ia + q;
+#pragma unroll
+      for (uint32_t r = 0; r < 4; ++r) cn += (q < na && r < nb && va[q] == vb[r]) ? 1u : 0u;
+    }
+    const int32_t amax = va[na - 1], bmax = vb[nb - 1];
+    if (amax <= bmax) ia += na;
+    if (bmax <= amax) ib += nb;
+  }
+  const bool sim = cstar != kNone ? cn >= cstar : predicate_exact(cn, rhs);
+  if (mirror) {
+    mpos = ed.y;
+  } else if (mpos == kNone) {
+    // the merge stopped before reaching x in a's row
+    uint32_t lo = ia, hi = ae;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (adj[mid] < (int32_t)x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    mpos = lo < ae && adj[lo] == (int32_t)x ? lo : kNone;
+  }
+  if (cn_out) {
+    cn_out[k] = cn;
+    if (mpos != kNone) cn_out[mpos] = cn;
+  } else {
+    const uint8_t f = sim ? kFlagY : kFlagN;
+    flag[k] = f;
+    if (mpos != kNone) flag[mpos] = f;
+  }
+}
+
+// Tiny owners (tiny_owner), listed by the planner.  Each wave takes 64
+// listed owners and spreads their edges over its lanes, one edge per lane
+// per round (a lane finds its owner by binary search over the owners' edge
+// counts, scanned into LDS), so a wave's time follows its longest edge, not
+// its busiest owner.  The list keeps the lanes on owners with work, where a
+// grid over all vertices left about one lane in twelve busy.
 __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t* __restrict__ rb,
                                                      const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ mirror, double eps,
@@ -264,69 +331,43 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t*
                                                      const uint32_t* __restrict__ tiny,
                                                      const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
                                                      uint32_t* __restrict__ cn_out, const uint64_t* __restrict__ cnt) {
+  __shared__ uint32_t s_incl[kBlock], s_x[kBlock], s_bb[kBlock], s_be[kBlock];
+  const uint32_t lane = lane_id(), w0 = (threadIdx.x / kWave) * kWave;
   const uint64_t count = cnt[kCntTiny];
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
-    const uint32_t x = tiny[i], owned = cnt_u[x];
-    const uint32_t bb = rb[x], be = re[x], l = be - bb;
-    for (uint32_t i = 0; i < owned; ++i) {
-      const uint4 ed = uedge[bb + i];  // (entry, reverse entry, row start of a, d(a))
-      const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
-      if (k >= m) continue;
-      const double rhs = predicate_rhs(eps, s, l);
-      const uint32_t cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
-      // merge in blocks of 4 entries per row: count the matches between the
-      // two current blocks, then advance the block with the smaller last
-      // value (both if equal).  A matching pair is current together exactly
-      // once, and cn + min(entries left in either row) bounds the final
-      // count, so the early exit is the tiered kernels'.
-      uint32_t ia = ab, ib = bb, cn = 0, mpos = kNone;
-      while (ia < ae && ib < be) {
-        if (cstar != kNone) {
-          const uint32_t rest = ae - ia < be - ib ? ae - ia : be - ib;
-          if (cn >= cstar || cn + rest < cstar) break;
-        }
-        const uint32_t na = ae - ia < 4 ? ae - ia : 4, nb = be - ib < 4 ? be - ib : 4;
-        int32_t va[4], vb[4];
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-          va[q] = q < na ? adj[ia + q] : 0;
-          vb[q] = q < nb ? adj[ib + q] : 0;
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-          if (q < na && va[q] == (int32_t)x) mpos = ia + q;
-#pragma unroll
-          for (uint32_t r = 0; r < 4; ++r) cn += (q < na && r < nb && va[q] == vb[r]) ? 1u : 0u;
-        }
-        const int32_t amax = va[na - 1], bmax = vb[nb - 1];
-        if (amax <= bmax) ia += na;
-        if (bmax <= amax) ib += nb;
-      }
-      const bool sim = cstar != kNone ? cn >= cstar : predicate_exact(cn, rhs);
-      if (mirror) {
-        mpos = ed.y;
-      } else if (mpos == kNone) {
-        // the merge stopped before reaching x in a's row
-        uint32_t lo = ia, hi = ae;
-        while (lo < hi) {
-          const uint32_t mid = lo + ((hi - lo) >> 1);
-          if (adj[mid] < (int32_t)x)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        mpos = lo < ae && adj[lo] == (int32_t)x ? lo : kNone;
-      }
-      if (cn_out) {
-        cn_out[k] = cn;
-        if (mpos != kNone) cn_out[mpos] = cn;
-      } else {
-        const uint8_t f = sim ? kFlagY : kFlagN;
-        flag[k] = f;
-        if (mpos != kNone) flag[mpos] = f;
-      }
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t g = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * kWave; g < count;
+       g += waves * kWave) {
+    uint32_t x = 0, owned = 0, bb = 0, be = 0;
+    if (g + lane < count) {
+      x = tiny[g + lane];
+      owned = cnt_u[x];
+      bb = rb[x];
+      be = re[x];
     }
+    const uint32_t incl = wave_incl_scan(owned);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+    // wave-private LDS rows: this wave's own lanes write and read them, in order
+    s_incl[w0 + lane] = incl;
+    s_x[w0 + lane] = x;
+    s_bb[w0 + lane] = bb;
+    s_be[w0 + lane] = be;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+      const uint32_t idx = e0 + lane;
+      if (idx >= total) continue;
+      // the owner: the first lane whose inclusive edge count exceeds idx
+      uint32_t lo = 0, hi = kWave - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_incl[w0 + mid] > idx)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const uint32_t first = lo ? s_incl[w0 + lo - 1] : 0u;
+      tiny_edge(m, s_x[w0 + lo], s_bb[w0 + lo], s_be[w0 + lo], idx - first, adj, mirror, eps, uedge, flag, cn_out);
+    }
+    __builtin_amdgcn_wave_barrier();  // every lane is done with the rows before the next group overwrites them
   }
 }
 
