@@ -116,7 +116,7 @@ class Tuning:
     small_unit_work: float = 262144.0
     large_unit_work: float = 4194304.0
     hash_tables: int = 1
-    tiny_work: int = 1024
+    tiny_work: int = 2048
     ms_similarity: float = 0.0
     ms_default: float = 0.0
     trials: int = 0

@@ -75,7 +75,7 @@ struct SimTuning {
   double small_unit_work = 262144.0;  // short-row elements per small-tier unit
   double large_unit_work = 4194304.0;  // per workgroup-tier unit
   uint32_t hash_tables = 1;  // 1: cuckoo hashes where they fit; 0: linear probing and bitmap windows only
-  uint32_t tiny_work = 1024;  // owners with sum over their edges of d(a) + d(b) <= this: one thread merges them
+  uint32_t tiny_work = 2048;  // owners with sum over their edges of d(a) + d(b) <= this: merged, one edge per lane
 };
 }  // namespace sgc
 

@@ -413,7 +413,7 @@ extern "C" int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* g, double eps, sgc_
     if ((rc = measure(t, &ms))) return rc;
     if (ms < cur_ms) cur = t, cur_ms = ms;
   }
-  const uint32_t tiny_work[] = {0, 512, 1024, 2048};
+  const uint32_t tiny_work[] = {0, 1024, 2048, 4096};
   for (uint32_t v : tiny_work) {
     SimTuning t = cur;
     t.tiny_work = v;

@@ -173,7 +173,7 @@ typedef struct {
                                 rows and runs of 2^20-id windows of <= 8192 keys in the workgroup tier);
                                 0: linear probing (small tier) and one bitmap window at a time only */
   uint32_t tiny_work;        /* owners whose edges' merge work, sum of d(a) + d(b), is at most this are
-                                decided by one thread each, without a work unit (0..65536; default 1024;
+                                decided by one thread each, without a work unit (0..65536; default 2048;
                                 0 turns this off) */
   double ms_similarity;      /* sgc_dgraph_autotune: similarity time with these knobs */
   double ms_default;         /* sgc_dgraph_autotune: similarity time with the defaults */

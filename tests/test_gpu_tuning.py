@@ -64,7 +64,7 @@ def test_autotune_keeps_a_candidate_and_results(ctx):
         assert best.small_unit_work in (65536.0, 262144.0, 1048576.0)
         assert best.large_unit_work in (1048576.0, 4194304.0, 16777216.0)
         assert best.hash_tables in (0, 1)
-        assert best.tiny_work in (0, 512, 1024, 2048)
+        assert best.tiny_work in (0, 1024, 2048, 4096)
         assert best.trials >= 12 and best.ms_similarity <= best.ms_default
         t = dg.tuning
         assert (t.small_max_degree, t.large_unit_work) == (best.small_max_degree, best.large_unit_work)
