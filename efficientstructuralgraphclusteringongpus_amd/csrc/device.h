@@ -39,7 +39,7 @@ enum Counter {
   kCntClaims,   // claims taken
   kCntDense,    // owned undecided entries listed for the one-wave-per-edge kernel
   kCntLabeled,  // entries of labelled rows (cores, borders): picks the classify pass's side
-  kCntTiny,     // tiny owners listed by the planner for the one-thread-per-owner kernel
+  kCntTiny,     // tiny owners listed by the planner for the tiny-owner kernel
   kCntOverflow, // owned undecided entries, when more than the unit array holds (asymmetric input)
   kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32

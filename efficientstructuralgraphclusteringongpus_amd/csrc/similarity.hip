@@ -165,9 +165,9 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
 // ---------------------------------------------------------------------------
 
 // Tiny owners: the merge work of all their edges, sum of d(a) + d(b), is at
-// most tiny_work.  They get no work unit; k_sim_tiny decides their edges with
-// one thread per owner, since a unit's fixed set-up would cost more than
-// their whole work.
+// most tiny_work.  They get no work unit; the planner lists them and
+// k_sim_tiny merges their edges, one edge per lane, since a unit's fixed
+// set-up would cost more than their whole work.
 __device__ __forceinline__ bool tiny_owner(uint32_t owned, uint64_t work, uint32_t l, uint32_t tiny_work) {
   return work + (uint64_t)owned * l <= tiny_work;
 }
