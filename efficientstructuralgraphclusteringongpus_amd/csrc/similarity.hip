@@ -36,7 +36,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kT1MaxDeg = 1024;  // small-tier tables are sized for this
 constexpr uint32_t kT1MaxEdges = 192;
-constexpr uint32_t kT2MaxEdges = 512;
+constexpr uint32_t kT2MaxEdges = 768;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t shift) {
   return (key * 0x9E3779B1u) >> shift;
@@ -641,8 +641,9 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
   // (c*, d(a), claim cap, slice end) of each edge: what a wave needs to size
   // a claim, in one 16-byte LDS read
   __shared__ __align__(16) uint4 e_q[T::kMaxEdges];
-  __shared__ uint32_t e_k[T::kMaxEdges], e_ab[T::kMaxEdges];
-  __shared__ uint32_t e_cur[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges], e_a[T::kMaxEdges];
+  // e_q.w also carries an edge's position between passes: where the next
+  // pass's slice may start (the row start before the first pass)
+  __shared__ uint32_t e_ab[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges], e_a[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend, s_fail;
   __shared__ uint4 s_next;  // the next unit's descriptor
   // the owner row's window boundaries, from the window-offset index
@@ -705,6 +706,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         key[j] = t < l ? (uint32_t)adj[bb + t] : 0u;
       }
     }
+    uint32_t my_k = kNone;  // this thread's edge entry (threads < ne), for the final writes
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
       const uint4 ed = uedge[un.x + t];  // (entry, reverse entry, row start of a, d(a))
@@ -717,11 +719,11 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         const double rhs = predicate_rhs(eps, s, l);
         cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
       }
-      e_k[t] = k;
+      my_k = k;
       e_a[t] = a;
       e_ab[t] = ab;
-      e_q[t].y = s;
-      e_cur[t] = ab;
+      e_q[t].y = s;  // 0 for a slot without an edge
+      e_q[t].w = ab;
       e_q[t].x = cstar;
       e_mir[t] = k < m ? ed.y : kNone;
       e_state[t] = 0ull;
@@ -769,7 +771,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         }
         // the edges' slices of this pass from the window-offset index, loaded
         // before the table is built so that their latency overlaps it
-        if (woff && !(hashed && whole) && threadIdx.x < ne && e_k[threadIdx.x] < m) {
+        if (woff && !(hashed && whole) && threadIdx.x < ne && e_q[threadIdx.x].y != 0) {
           const uint32_t* wo = woff + (size_t)e_a[threadIdx.x] * wstride + (wlo >> kWindowBits);
           pre_lo = wo[0];
           pre_hi = wo[wspan];
@@ -855,7 +857,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       const ClaimTable cc{adj, tab, wlo, whi, mask, shift, b, last || exact, mir_here};
       if (threadIdx.x < ne) {
         const uint32_t t = threadIdx.x;
-        const uint32_t cur = e_cur[t], ab = e_ab[t], s = e_q[t].y;
+        const uint32_t cur = e_q[t].w, ab = e_ab[t], s = e_q[t].y;
         e_pos[t] = cur;
         e_q[t].w = ab + s;
         uint32_t cap = T::kChunk;
@@ -863,7 +865,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           // the edge's slice of this window; the elements skipped since the
           // last window lie in windows without owner keys: consumed, no hits
           uint32_t lo = cur, hi = cur;
-          if (e_k[t] < m) {
+          if (s != 0) {
             lo = pre_lo;
             hi = pre_hi;
             if (lo > cur) e_state[t] += (unsigned long long)(lo - cur);
@@ -947,16 +949,14 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       // (every thread read s_next before the table barriers)
       if (threadIdx.x == 0 && kend >= be) fetch_next();
       SGC_TICK(2);
-      __syncthreads();
+      __syncthreads();  // all streaming done: the table may be rebuilt, e_q.w holds each slice's end
       SGC_TICK(3);
-      if (threadIdx.x < ne) e_cur[threadIdx.x] = e_q[threadIdx.x].w;
       kcur = kend;
-      __syncthreads();
     }
     // ---- decide and write both directed entries of every edge
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
-      const uint32_t k = e_k[t];
+      const uint32_t k = my_k;
       if (k < m) {
         const uint32_t hits = (uint32_t)(e_state[t] >> 32);
         const uint32_t cstar = e_q[t].x;
