@@ -114,7 +114,7 @@ class _Tuning(C.Structure):
 class Tuning:
     small_max_degree: int = 1024
     small_unit_work: float = 262144.0
-    large_unit_work: float = 4194304.0
+    large_unit_work: float = 8388608.0
     hash_tables: int = 1
     tiny_work: int = 2048
     ms_similarity: float = 0.0

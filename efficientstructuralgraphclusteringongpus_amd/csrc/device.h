@@ -73,7 +73,7 @@ namespace sgc {
 struct SimTuning {
   uint32_t small_max_degree = 1024;  // owners up to this degree take the small tier (<= 1024)
   double small_unit_work = 262144.0;  // short-row elements per small-tier unit
-  double large_unit_work = 4194304.0;  // per workgroup-tier unit
+  double large_unit_work = 8388608.0;  // per workgroup-tier unit
   uint32_t hash_tables = 1;  // 1: cuckoo hashes where they fit; 0: linear probing and bitmap windows only
   uint32_t tiny_work = 2048;  // owners with sum over their edges of d(a) + d(b) <= this: merged, one edge per lane
 };

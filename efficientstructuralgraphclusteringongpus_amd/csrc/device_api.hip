@@ -379,7 +379,7 @@ extern "C" int sgc_dgraph_autotune(sgc_ctx* ctx, sgc_dgraph* g, double eps, sgc_
   if (rc) return rc;
   SimTuning cur = defaults;
   double cur_ms = ms_default;
-  const double large_work[] = {1048576.0, 4194304.0, 16777216.0};
+  const double large_work[] = {2097152.0, 8388608.0, 33554432.0};
   const uint32_t small_deg[] = {128, 256, 512, 1024};
   const double small_work[] = {65536.0, 262144.0, 1048576.0};
   for (double v : large_work) {

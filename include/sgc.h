@@ -168,7 +168,7 @@ void sgc_dgraph_free(sgc_dgraph* dg);
 typedef struct {
   uint32_t small_max_degree; /* owners of degree <= this take the small tier (1..1024; default 1024) */
   double small_unit_work;    /* short-row elements per small-tier work unit (default 262144) */
-  double large_unit_work;    /* per workgroup-tier work unit (default 4194304) */
+  double large_unit_work;    /* per workgroup-tier work unit (default 8388608) */
   uint32_t hash_tables;      /* 1 (default): owner rows as cuckoo hashes where they fit (small tier; whole
                                 rows and runs of 2^20-id windows of <= 8192 keys in the workgroup tier);
                                 0: linear probing (small tier) and one bitmap window at a time only */

@@ -62,7 +62,7 @@ def test_autotune_keeps_a_candidate_and_results(ctx):
         say("tuned", best)
         assert best.small_max_degree in (128, 256, 512, 1024)
         assert best.small_unit_work in (65536.0, 262144.0, 1048576.0)
-        assert best.large_unit_work in (1048576.0, 4194304.0, 16777216.0)
+        assert best.large_unit_work in (2097152.0, 8388608.0, 33554432.0)
         assert best.hash_tables in (0, 1)
         assert best.tiny_work in (0, 1024, 2048, 4096)
         assert best.trials >= 12 and best.ms_similarity <= best.ms_default
