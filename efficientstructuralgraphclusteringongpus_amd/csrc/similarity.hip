@@ -35,7 +35,7 @@ constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kT1MaxDeg = 1024;  // small-tier tables are sized for this
-constexpr uint32_t kT1MaxEdges = 192;
+constexpr uint32_t kT1MaxEdges = 256;
 constexpr uint32_t kT2MaxEdges = 768;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t shift) {
