@@ -30,11 +30,13 @@ namespace sgc {
 
 namespace {
 
-
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 constexpr uint32_t kT1MaxDeg = 1024;  // small-tier tables are sized for this
+// Edges per unit, bounded by LDS at 40 B per edge: six 256-thread small-tier
+// workgroups of 25.6 KB share a CU; a workgroup-tier unit's 768 edges sit
+// beside its 128 KB table (162 KB in all).
 constexpr uint32_t kT1MaxEdges = 256;
 constexpr uint32_t kT2MaxEdges = 768;
 
