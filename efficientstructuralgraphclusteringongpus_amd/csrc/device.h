@@ -41,6 +41,7 @@ enum Counter {
   kCntLabeled,  // entries of labelled rows (cores, borders): picks the classify pass's side
   kCntTiny,     // tiny owners listed by the planner for the tiny-owner kernel
   kCntOverflow, // owned undecided entries, when more than the unit array holds (asymmetric input)
+  kCntMarked,   // cores marked by the core pass: none lets the later edge passes return at once
   kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
   kCntSlots = 32
 };

@@ -274,15 +274,17 @@ __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t
   if (mu_dev) mu = *mu_dev;
   const uint32_t thr = mu >= 1 ? (uint32_t)mu : 1u;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  uint64_t labeled = 0;
+  uint64_t labeled = 0, marked = 0;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     const bool c = sd[x] >= thr;
     core[x] = c;
     parent[x] = (int32_t)x;
     label[x] = kNoLabel;
     if (c && rb) labeled += re[x] - rb[x];
+    marked += c;
   }
   block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
+  block_atomic_add((unsigned long long*)&cnt[kCntMarked], marked);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,7 +314,9 @@ __device__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
 
 __global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __restrict__ src,
                                                   const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
-                                                  const uint8_t* __restrict__ core, int32_t* parent) {
+                                                  const uint8_t* __restrict__ core, int32_t* parent,
+                                                  const uint64_t* __restrict__ cnt) {
+  if (cnt[kCntMarked] == 0) return;  // no core: no core-core edge to join
   constexpr uint32_t C = kEdgeChunks;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
@@ -354,6 +358,7 @@ __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* _
                                                    const uint8_t* __restrict__ core, const uint32_t* __restrict__ rb,
                                                    const uint32_t* __restrict__ re, int32_t* label,
                                                    uint64_t* __restrict__ cnt) {
+  if (cnt[kCntMarked] == 0) return;  // no core: no border
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
@@ -409,6 +414,8 @@ __global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, uint64_t m_all,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ label, int32_t* __restrict__ lo,
                                                      int32_t* __restrict__ hi, const uint64_t* __restrict__ cnt) {
+  // no core means no label anywhere: lo and hi keep their initial values
+  if (cnt[kCntMarked] == 0) return;
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
@@ -561,7 +568,8 @@ void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, h
 }
 
 void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  if (g->m) k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent);
+  if (g->m)
+    k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent, b.cnt);
 }
 
 void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
