@@ -1,10 +1,14 @@
 // C-ABI: device contexts, device-resident graphs and the plain SCAN pass.
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <mutex>
 #include <vector>
 
+#include "GPUScan.h"
 #include "device.h"
 #include "graph_host.h"
 
@@ -497,24 +501,45 @@ extern "C" int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms) {
 }
 
 // ---------------------------------------------------------------------------
-// Reference-compatible entry point
+// Reference-compatible entry points
 // ---------------------------------------------------------------------------
 
 namespace {
 std::mutex g_compat_mu;
 sgc_ctx* g_compat_ctx = nullptr;
-}  // namespace
 
-extern "C" int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v, double eps, int32_t miu) {
-  (void)Edge_u;
-  (void)Edge_v;
+// The reference's error contract (GPUSCAN++/GPUScan.cu:24-32): a
+// "GPUassert: <message> <file> <line>" line on stderr, then exit with a
+// non-zero code.  The library's own entry points return status codes; only
+// the two drop-ins below end the process.
+[[noreturn]] void compat_abort(int code, int line) {
+  fprintf(stderr, "GPUassert: %s %s %d\n", sgc_last_error(), __FILE__, line);
+  fflush(stderr);
+  exit(code ? code : 1);
+}
+
+#define SGC_COMPAT_CHECK(expr)                  \
+  do {                                          \
+    const int rc_ = (expr);                     \
+    if (rc_ != SGC_OK) compat_abort(rc_, __LINE__); \
+  } while (0)
+
+// One plain pass over the caller's reference-layout graph on device 0;
+// returns the malloc'd labels or ends the process.
+int32_t* compat_scan(const sgc_graph_t* g, double eps, int32_t miu) {
   clear_error();
-  if (!g || g->n < 0 || g->m < 0 || (g->m && (!g->adj || !g->num_edges))) {
-    fail(SGC_EINVAL, "invalid graph_t");
-    return nullptr;
+  if (!g) {
+    fail(SGC_EINVAL, "graph_t is null");
+    compat_abort(SGC_EINVAL, __LINE__);
   }
-  std::lock_guard<std::mutex> lock(g_compat_mu);
-  if (!g_compat_ctx && sgc_ctx_create(0, &g_compat_ctx) != SGC_OK) return nullptr;
+  if (g->n < 0 || g->m < 0 || g->n >= (1l << 31) || g->m >= (1l << 32)) {
+    fail(SGC_EINVAL, "graph_t has n=%ld, m=%ld (need 0 <= n < 2^31, 0 <= m < 2^32)", g->n, g->m);
+    compat_abort(SGC_EINVAL, __LINE__);
+  }
+  if (!g->num_edges || (g->m && !g->adj)) {
+    fail(SGC_EINVAL, "graph_t has a null %s", !g->num_edges ? "num_edges" : "adj");
+    compat_abort(SGC_EINVAL, __LINE__);
+  }
   sgc_host_graph hg;
   hg.n = (uint64_t)g->n;
   hg.m = (uint64_t)g->m;
@@ -522,24 +547,54 @@ extern "C" int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v
   for (uint64_t u = 0; u <= hg.n; ++u) rp[u] = g->num_edges[u];
   hg.row_ptr = rp.data();
   hg.adj = g->adj;
+  // the device kernels trust the CSR: check the input contract
+  // (GPUSCAN++/Graph.cpp:270-288 and ids < n) before anything runs
+  int rc = validate_graph(&hg, false);
   sgc_dgraph* dg = nullptr;
-  int rc = sgc_dgraph_upload(g_compat_ctx, &hg, &dg);
+  int32_t* out = nullptr;
+  if (rc == SGC_OK) {
+    std::lock_guard<std::mutex> lock(g_compat_mu);
+    if (!g_compat_ctx) rc = sgc_ctx_create(0, &g_compat_ctx);
+    if (rc == SGC_OK) rc = sgc_dgraph_upload(g_compat_ctx, &hg, &dg);
+    if (rc == SGC_OK) {
+      out = (int32_t*)malloc(std::max<size_t>(hg.n, 1) * sizeof(int32_t));
+      if (!out) rc = fail(SGC_ENOMEM, "malloc of %llu labels failed", (unsigned long long)hg.n);
+    }
+    sgc_stats st;
+    if (rc == SGC_OK) rc = sgc_scan(g_compat_ctx, dg, eps, miu, out, nullptr, nullptr, &st);
+    if (dg) sgc_dgraph_free(dg);
+    if (rc == SGC_OK) {
+      // the reference's phase lines (GPUSCAN++/GPUScan.cu:890-989)
+      fprintf(stderr, "\nPrep time: %.3lf s\n", st.ms_prep / 1000.0);
+      fprintf(stderr, "\nCheck Core time: %.3lf s\n", (st.ms_similarity + st.ms_core) / 1000.0);
+      fprintf(stderr, "core number: %llu\n", (unsigned long long)st.cores);
+      fprintf(stderr, "\nFirst Link Phase time: %.3lf s\n", st.ms_cluster / 1000.0);
+      fprintf(stderr, "Cluster non-core time: %.3lf s\n", st.ms_border / 1000.0);
+      fprintf(stderr, "\nClassify Others time: %.3lf s\n", st.ms_classify / 1000.0);
+    }
+  }
   hg.row_ptr = nullptr;
   hg.adj = nullptr;
-  if (rc) return nullptr;
-  int32_t* out = (int32_t*)malloc(std::max<size_t>(hg.n, 1) * sizeof(int32_t));
-  sgc_stats st;
-  rc = out ? sgc_scan(g_compat_ctx, dg, eps, miu, out, nullptr, nullptr, &st) : fail(SGC_ENOMEM, "malloc failed");
-  sgc_dgraph_free(dg);
-  if (rc) {
+  if (rc != SGC_OK) {
     free(out);
-    return nullptr;
+    compat_abort(rc, __LINE__);
   }
-  fprintf(stderr, "\nPrep time: %.3lf s\n", st.ms_prep / 1000.0);
-  fprintf(stderr, "\nCheck Core time: %.3lf s\n", (st.ms_similarity + st.ms_core) / 1000.0);
-  fprintf(stderr, "core number: %llu\n", (unsigned long long)st.cores);
-  fprintf(stderr, "\nFirst Link Phase time: %.3lf s\n", st.ms_cluster / 1000.0);
-  fprintf(stderr, "Cluster non-core time: %.3lf s\n", st.ms_border / 1000.0);
-  fprintf(stderr, "\nClassify Others time: %.3lf s\n", st.ms_classify / 1000.0);
   return out;
+}
+}  // namespace
+
+extern "C" int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v, double eps, int32_t miu) {
+  (void)Edge_u;
+  (void)Edge_v;
+  return compat_scan(g, eps, miu);
+}
+
+// C++ linkage, the reference's own symbol (GPUSCAN++/GPUScan.h:7).
+int* GPUScan(graph_t* g, int* Edge_u, int* Edge_v, double eps, int miu) {
+  static_assert(sizeof(graph_t) == sizeof(sgc_graph_t), "graph_t layout");
+  static_assert(offsetof(graph_t, num_edges) == offsetof(sgc_graph_t, num_edges), "graph_t layout");
+  static_assert(offsetof(graph_t, eid) == offsetof(sgc_graph_t, eid), "graph_t layout");
+  (void)Edge_u;
+  (void)Edge_v;
+  return compat_scan(reinterpret_cast<const sgc_graph_t*>(g), eps, miu);
 }

@@ -8,6 +8,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -104,16 +105,29 @@ int validate_graph(const sgc_host_graph* g, bool check_symmetry) {
     return fail(SGC_EFORMAT, "row offsets do not span the adjacency array (row_ptr[0]=%llu, row_ptr[n]=%llu, m=%llu)",
                 (unsigned long long)g->row_ptr[0], (unsigned long long)g->row_ptr[n],
                 (unsigned long long)g->m);
-  // 0 ok, 1 self loop, 2 unsorted, 3 id out of range, 4 asymmetric, 5 bad offsets
-  int err = 0;
+  // 0 ok, 1 self loop, 2 unsorted, 3 id out of range, 4 asymmetric, 5 bad offsets.
+  // `err` is read by every thread to stop early and set once by the first
+  // failing row: an atomic, so the early-exit reads race with nothing; the
+  // (code, row) pair itself is stored under the critical section.
+  std::atomic<int> err{0};
+  int err_code = 0;
   uint64_t err_u = 0;
+  auto report = [&](int code, uint64_t u) {
+#pragma omp critical(sgc_validate)
+    {
+      if (!err_code) {
+        err_code = code;
+        err_u = u;
+        err.store(code, std::memory_order_relaxed);
+      }
+    }
+  };
 #pragma omp parallel for schedule(dynamic, 4096)
   for (uint64_t u = 0; u < n; ++u) {
-    if (err) continue;
+    if (err.load(std::memory_order_relaxed)) continue;
     uint64_t b = g->row_ptr[u], e = g->row_ptr[u + 1];
     if (e < b || e > g->m) {
-#pragma omp critical
-      { if (!err) { err = 5; err_u = u; } }
+      report(5, u);
       continue;
     }
     for (uint64_t k = b; k < e; ++k) {
@@ -131,13 +145,12 @@ int validate_graph(const sgc_host_graph* g, bool check_symmetry) {
         if (p == ve || g->adj[p] != (int32_t)u) code = 4;
       }
       if (code) {
-#pragma omp critical
-        { if (!err) { err = code; err_u = u; } }
+        report(code, u);
         break;
       }
     }
   }
-  switch (err) {
+  switch (err_code) {
     case 1: return fail(SGC_EFORMAT, "Self loop (vertex %llu)", (unsigned long long)err_u);
     case 2: return fail(SGC_EFORMAT, "Edges not sorted in increasing id order! (vertex %llu)", (unsigned long long)err_u);
     case 3: return fail(SGC_EFORMAT, "neighbour id out of range [0, n) in row %llu", (unsigned long long)err_u);
@@ -152,16 +165,19 @@ int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint
   *out = nullptr;
   if (n == 0 || n > (1ull << 31)) return fail(SGC_EINVAL, "vertex count %llu outside [1, 2^31]", (unsigned long long)n);
   std::vector<uint64_t> cnt(n + 1, 0);
-  int bad = 0;
+  std::atomic<int> bad{0};
 #pragma omp parallel for schedule(static)
   for (uint64_t i = 0; i < count; ++i) {
     uint32_t u = src[i], v = dst[i];
-    if (u >= n || v >= n) { bad = 1; continue; }
+    if (u >= n || v >= n) {
+      bad.store(1, std::memory_order_relaxed);
+      continue;
+    }
     if (u == v) continue;
     __atomic_fetch_add(&cnt[u], 1, __ATOMIC_RELAXED);
     __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
   }
-  if (bad) return fail(SGC_EINVAL, "edge endpoint >= n");
+  if (bad.load()) return fail(SGC_EINVAL, "edge endpoint >= n");
   progress("graph build: degrees counted (%llu pairs)", (unsigned long long)count);
   std::vector<uint64_t> off(n + 1);
   off[0] = 0;

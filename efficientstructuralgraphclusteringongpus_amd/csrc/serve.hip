@@ -165,6 +165,7 @@ struct sgc_server {
   uint32_t next_graph = 1;
   uint64_t next_ticket = 1;
   hipStream_t streams[sgc::kStreams] = {};
+  hipStream_t capture_stream = nullptr;  // query graphs are captured here, at admission
   int next_stream = 0;
   std::unordered_map<uint32_t, sgc::Graph*> graphs;
   std::deque<sgc::Query> pending;
@@ -213,9 +214,11 @@ void enqueue_query(const Graph* G, const Slot& sl, const ScanParams* p, hipStrea
   (void)hipMemcpyAsync(sl.h_cnt, sl.pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, st);
 }
 
-// Captures the slot's query work into an executable graph.
+// Captures the slot's query work into an executable graph.  Capture runs at
+// admission on the server's own capture stream, which carries no other work:
+// never on a serving stream, where other graphs' queries may be running.
 int capture_slot(sgc_server* S, const Graph* G, Slot& sl) {
-  hipStream_t st = S->streams[sl.stream];
+  hipStream_t st = S->capture_stream;
   SGC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   enqueue_query(G, sl, nullptr, st);
   const hipError_t le = hipGetLastError();
@@ -237,10 +240,6 @@ int launch_query(sgc_server* S, Graph* G, Slot& sl, const Query& q) {
   hipStream_t st = S->streams[sl.stream];
   SGC_HIP(hipEventRecord(sl.start, st));
   if (S->graph_launch) {
-    if (!sl.exec) {
-      const int rc = capture_slot(S, G, sl);
-      if (rc) return rc;
-    }
     sl.h_params->eps = q.eps;
     sl.h_params->mu = q.mu;
     SGC_HIP(hipMemcpyAsync(sl.d_params, sl.h_params, sizeof(QueryParams), hipMemcpyHostToDevice, st));
@@ -346,8 +345,9 @@ extern "C" int sgc_server_create(sgc_ctx* ctx, uint64_t hbm_budget, uint32_t slo
   S->ctx = ctx;
   S->budget = hbm_budget;
   S->slots_per_graph = slots_per_graph;
-  for (int i = 0; i < kStreams; ++i) {
-    if (hipStreamCreateWithFlags(&S->streams[i], hipStreamNonBlocking) != hipSuccess) {
+  for (int i = 0; i <= kStreams; ++i) {
+    hipStream_t* st = i < kStreams ? &S->streams[i] : &S->capture_stream;
+    if (hipStreamCreateWithFlags(st, hipStreamNonBlocking) != hipSuccess) {
       for (int j = 0; j < i; ++j) (void)hipStreamDestroy(S->streams[j]);
       delete S;
       return fail(SGC_EDEVICE, "stream creation failed");
@@ -363,6 +363,7 @@ extern "C" void sgc_server_destroy(sgc_server* S) {
   for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(S->streams[i]);
   for (auto& kv : S->graphs) free_graph(kv.second);
   for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(S->streams[i]);
+  if (S->capture_stream) (void)hipStreamDestroy(S->capture_stream);
   delete S;
 }
 
@@ -468,6 +469,12 @@ extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uin
       (void)hipGetLastError();
       free_graph(G);
       return fail(SGC_ENOMEM, "query slot set-up failed");
+    }
+    // the slot's query work as one replayable graph (used when graph launch
+    // is on), captured now on the idle capture stream
+    if ((rc = capture_slot(S, G, sl))) {
+      free_graph(G);
+      return rc;
     }
   }
   G->id = S->next_graph++;

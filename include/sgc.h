@@ -351,11 +351,17 @@ typedef struct {
   uint32_t* eid;
 } sgc_graph_t;
 
-/* Drop-in for int* GPUScan(graph_t* g, int* Edge_u, int* Edge_v, double eps,
- * int miu) (GPUSCAN++/GPUScan.h:7).  Returns a malloc'd int32[n] the caller
- * frees, or NULL on error (message in sgc_last_error()).  Edge_u/Edge_v/eid
- * are accepted for signature compatibility and are not read: the device
- * pass orients edges itself. */
+/* C-linkage twin of the drop-in int* GPUScan(graph_t* g, int* Edge_u,
+ * int* Edge_v, double eps, int miu) (GPUSCAN++/GPUScan.h:7), which
+ * libsgc.so also exports with the reference's C++ linkage (GPUScan.h in this
+ * include folder).  Both keep the reference's contract and are the only entry
+ * points that end the process: they return a malloc'd int32[n] the caller
+ * frees, and on any error (null or out-of-range graph_t, null num_edges or
+ * adj, a CSR violating the input contract, no device, allocation or HIP
+ * failure) print "GPUassert: <message> <file> <line>" on stderr and exit
+ * with the SGC_* status as the exit code (GPUSCAN++/GPUScan.cu:24-32).
+ * Edge_u/Edge_v/eid are accepted for signature compatibility and are not
+ * read: the device pass orients edges itself. */
 int32_t* sgc_GPUScan(sgc_graph_t* g, int32_t* Edge_u, int32_t* Edge_v, double eps, int32_t miu);
 
 #ifdef __cplusplus

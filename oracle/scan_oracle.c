@@ -366,3 +366,116 @@ uint64_t orc_similarity_entries(uint64_t n, const uint64_t* rp, const int32_t* a
   }
   return similar;
 }
+
+int orc_labels_from_sim(uint64_t n, const uint64_t* rp, const int32_t* adj, const uint8_t* sim, int32_t mu,
+                        int32_t* labels, uint8_t* core, orc_stats* st) {
+  const uint64_t m = rp[n];
+  if (m >= (1ull << 32)) return -1;
+  orc_stats s;
+  memset(&s, 0, sizeof(s));
+  s.edges = m / 2;
+  int32_t* role = labels;
+  uint64_t similar = 0;
+  /* core check: the state machine's fixed point once every edge is known:
+   * sd = #similar, ed = degree - #dissimilar = #similar, so core <=>
+   * #similar >= mu (mu <= 0: no vertex starts at -2, one similar edge
+   * makes a core, every other vertex stays -1). */
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : similar)
+  for (uint64_t x = 0; x < n; ++x) {
+    int32_t y = 0;
+    for (uint64_t k = rp[x]; k < rp[x + 1]; ++k) y += sim[k] != 0;
+    similar += (uint64_t)y;
+    const int is_core = mu >= 1 ? y >= mu : y >= 1;
+    role[x] = is_core ? (int32_t)x : (mu >= 1 ? -2 : -1);
+  }
+  s.similar_edges = similar / 2;
+  uint64_t ncore = 0;
+  for (uint64_t x = 0; x < n; ++x) ncore += role[x] >= 0;
+  s.cores = ncore;
+  if (core)
+    for (uint64_t x = 0; x < n; ++x) core[x] = role[x] >= 0;
+  int32_t* cores = (int32_t*)malloc((ncore ? ncore : 1) * sizeof(int32_t));
+  ncore = 0;
+  for (uint64_t x = 0; x < n; ++x)
+    if (role[x] >= 0) cores[ncore++] = (int32_t)x;
+  /* clusters: similar core-core edges (GPUScan.cu:479-630), then flatten
+   * to the minimum core id (:632-652) */
+  for (uint64_t i = 0; i < ncore; ++i) {
+    int32_t u = cores[i];
+    for (uint64_t k = rp[u]; k < rp[u + 1]; ++k) {
+      int32_t v = adj[k];
+      if (v >= u) break;
+      if (role[v] >= 0 && sim[k]) unite(role, u, v);
+    }
+  }
+  for (uint64_t i = 0; i < ncore; ++i) role[cores[i]] = find_root(role, cores[i]);
+  for (uint64_t i = 0; i < ncore; ++i) s.clusters += role[cores[i]] == cores[i];
+  /* borders: minimum candidate cluster over similar edges to cores */
+  int32_t* cand = (int32_t*)malloc((n ? n : 1) * sizeof(int32_t));
+  for (uint64_t x = 0; x < n; ++x) cand[x] = INT32_MAX;
+  for (uint64_t i = 0; i < ncore; ++i) {
+    int32_t u = cores[i];
+    for (uint64_t k = rp[u]; k < rp[u + 1]; ++k) {
+      int32_t v = adj[k];
+      if (role[v] < 0 && sim[k] && role[u] < cand[v]) cand[v] = role[u];
+    }
+  }
+  for (uint64_t x = 0; x < n; ++x)
+    if (role[x] < 0 && cand[x] != INT32_MAX) {
+      role[x] = cand[x];
+      ++s.borders;
+    }
+  /* hubs and outliers (GPUScan.cu:732-798): a -2 vertex of degree >= 2 whose
+   * neighbours carry two different labels >= 0 becomes -1 */
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (uint64_t x = 0; x < n; ++x) {
+    if (role[x] != -2) continue;
+    uint64_t b = rp[x], e = rp[x + 1];
+    if (e - b <= 1) continue;
+    int32_t first = -1;
+    for (uint64_t p = b; p < e; ++p) {
+      /* a neighbour's final label: cores and borders are settled above, and
+       * no vertex turned hub in this loop is read as a label (-1 < 0) */
+      int32_t r = role[adj[p]];
+      if (r < 0) continue;
+      if (first < 0) {
+        first = r;
+      } else if (r != first) {
+        cand[x] = -1; /* decided after the loop, so every read sees final labels */
+        break;
+      }
+    }
+  }
+  for (uint64_t x = 0; x < n; ++x)
+    if (role[x] == -2 && cand[x] == -1) role[x] = -1;
+  for (uint64_t x = 0; x < n; ++x) {
+    s.hubs += role[x] == -1;
+    s.outliers += role[x] == -2;
+  }
+  if (st) *st = s;
+  free(cores);
+  free(cand);
+  return 0;
+}
+
+void orc_prep_counts(uint64_t n, const uint64_t* rp, const int32_t* adj, double eps, uint64_t* pn, uint64_t* py,
+                     uint64_t* pu) {
+  uint64_t cn = 0, cy = 0, cu = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : cn, cy, cu)
+  for (uint64_t x = 0; x < n; ++x) {
+    const uint64_t dx = deg_of(rp, x);
+    for (uint64_t k = rp[x]; k < rp[x + 1]; ++k) {
+      const uint64_t v = (uint64_t)adj[k], dv = deg_of(rp, v);
+      /* each undirected edge once, from its lower (degree, id) endpoint a
+       * (GPUSCAN++/Graph.cpp:75-83) */
+      if (!(dv > dx || (dv == dx && v > x))) continue;
+      const int p = orc_prep((uint32_t)dx, (uint32_t)dv, eps);
+      cn += p == ORC_N;
+      cy += p == ORC_Y;
+      cu += p == ORC_U;
+    }
+  }
+  *pn = cn;
+  *py = cy;
+  *pu = cu;
+}

@@ -57,6 +57,24 @@ uint64_t orc_similarity_entries(uint64_t n, const uint64_t* row_ptr, const int32
                                 double eps, const uint64_t* entry, uint64_t count, uint8_t* out,
                                 int threads);
 
+/* The phases after the similarity phase (GPUSCAN++/GPUScan.cu:276-798)
+ * given the final similarity of every directed entry (sim[m], 1 = similar,
+ * symmetric): core flags (the fixed point of the sd/ed state machine,
+ * :316-363), clusters by union-find over similar core-core edges labelled
+ * by minimum core id (:465-652), borders by the canonical minimum candidate
+ * (:654-730), hubs and outliers (:732-798).  Same outputs as orc_scan when
+ * sim is orc_scan's own.  For checking a device pass at sizes where
+ * orc_scan's sequential similarity is too slow.  stats: cores, clusters,
+ * borders, hubs, outliers, similar_edges.  Returns 0, or -1 if m >= 2^32. */
+int orc_labels_from_sim(uint64_t n, const uint64_t* row_ptr, const int32_t* adj, const uint8_t* sim,
+                        int32_t mu, int32_t* labels, uint8_t* core, orc_stats* stats);
+
+/* Degree-only pre-decision counts over every undirected edge
+ * (GPUSCAN++/GPUScan.cu:133-172), OpenMP-parallel: edges decided N, Y and
+ * left undecided (U). */
+void orc_prep_counts(uint64_t n, const uint64_t* row_ptr, const int32_t* adj, double eps, uint64_t* prep_n,
+                     uint64_t* prep_y, uint64_t* prep_u);
+
 #ifdef __cplusplus
 }
 #endif

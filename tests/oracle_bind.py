@@ -48,6 +48,11 @@ def oracle():
     lib.orc_scan.argtypes = [C.c_uint64, _VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, C.POINTER(OrcStats)]
     lib.orc_similarity_entries.restype = C.c_uint64
     lib.orc_similarity_entries.argtypes = [C.c_uint64, _VP, _VP, C.c_double, _VP, C.c_uint64, _VP, C.c_int]
+    lib.orc_labels_from_sim.restype = C.c_int
+    lib.orc_labels_from_sim.argtypes = [C.c_uint64, _VP, _VP, _VP, C.c_int32, _VP, _VP, C.POINTER(OrcStats)]
+    lib.orc_prep_counts.restype = None
+    lib.orc_prep_counts.argtypes = [C.c_uint64, _VP, _VP, C.c_double, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     _orc = lib
     return lib
 
@@ -91,6 +96,34 @@ def orc_similarity_entries(row_ptr, adj, eps, entries, threads=0):
     y = oracle().orc_similarity_entries(len(rp) - 1, _p(rp), _p(ad), float(eps), _p(en), len(en), _p(out),
                                         int(threads))
     return int(y), out[: len(en)]
+
+
+def orc_labels_from_sim(row_ptr, adj, sim, mu: int):
+    """The oracle's phases after similarity, over a given flag per directed
+    entry: (labels int32[n], core uint8[n], stats dict)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    ad = np.ascontiguousarray(adj, dtype=np.int32)
+    sm = np.ascontiguousarray(sim, dtype=np.uint8)
+    n, m = len(rp) - 1, len(ad)
+    assert len(sm) == m
+    labels = np.empty(max(n, 1), dtype=np.int32)
+    core = np.empty(max(n, 1), dtype=np.uint8)
+    st = OrcStats()
+    z = np.zeros(1, np.int32)
+    rc = oracle().orc_labels_from_sim(n, _p(rp), _p(ad if m else z), _p(sm if m else z.view(np.uint8)), int(mu),
+                                      _p(labels), _p(core), C.byref(st))
+    assert rc == 0
+    return labels[:n], core[:n], st.as_dict()
+
+
+def orc_prep_counts(row_ptr, adj, eps: float):
+    """(N, Y, U) counts of the degree-only pre-decision over every edge."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    ad = np.ascontiguousarray(adj, dtype=np.int32)
+    a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    oracle().orc_prep_counts(len(rp) - 1, _p(rp), _p(ad if len(ad) else np.zeros(1, np.int32)), float(eps),
+                             C.byref(a), C.byref(b), C.byref(c))
+    return int(a.value), int(b.value), int(c.value)
 
 
 # ---------------------------------------------------------------------------

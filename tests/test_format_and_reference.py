@@ -173,3 +173,29 @@ def test_reference_rejects_what_loader_rejects(tmp_path):
         with pytest.raises(sgc.SGCError) as ei:
             sgc.HostGraph.load(d)
         assert needle in ei.value.message
+
+
+@pytest.mark.parametrize("degs,m_field,code,needle", [
+    # degree sum exactly 2^32: the header holds 0 (the 64-bit extension), so
+    # the header is accepted and the loader asks for 2^34 bytes of b_adj.bin
+    ([2**31 - 1, 2**31 - 1, 2], 0, 2, "expected 17179869184"),
+    # degree sum 2^32 + 7: header 7
+    ([2**31 - 1, 2**31 - 1, 9], 7, 2, "expected 17179869212"),
+    # a header that is not the degree sum mod 2^32 is refused before any read
+    ([2**31 - 1, 2**31 - 1, 2], 5, 3, "does not match the degree sum 4294967296"),
+    # the reference's own reading (a 32-bit m equal to the sum) below 2^32
+    ([2**31 - 1, 3], 2**31 + 2, 2, "expected 8589934600"),
+])
+def test_header_64bit_extension(tmp_path, degs, m_field, code, needle):
+    """b_degree.bin's uint32 m field holds sum(deg) mod 2^32 when the sum
+    reaches 2^32 (the reference writes m as 4 bytes,
+    csr_convert/EdgeListToBinsConverter.cpp:103-107, and cannot represent
+    it).  The header arithmetic is checked before b_adj.bin is read, so
+    degree arrays summing past 2^32 with a short adjacency file pin it
+    without a 16 GB file."""
+    d = str(tmp_path / "g")
+    write_raw(d, len(degs), degs, [1, 0], m_field=m_field)
+    with pytest.raises(sgc.SGCError) as ei:
+        sgc.HostGraph.load(d)
+    assert ei.value.code == code, ei.value.message
+    assert needle in ei.value.message, ei.value.message

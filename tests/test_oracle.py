@@ -25,6 +25,8 @@ import pytest
 import bruteforce as bf
 import oracle_bind as ob
 
+import efficientstructuralgraphclusteringongpus_amd as sgc
+
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 EPS_GRID = [round(0.05 * i, 2) for i in range(1, 21)] + [0.0, 0.33, 1.25]
 
@@ -192,3 +194,57 @@ def test_similarity_entries_match_scan():
         y, out = ob.orc_similarity_entries(rp, adj, eps, entries, threads=2)
         assert np.array_equal(out, sim)
         assert y == int(sim.sum())
+
+
+def test_labels_from_sim_matches_scan():
+    """orc_labels_from_sim (the phases after similarity, over given flags)
+    against orc_scan on the flags orc_scan itself computes: random graphs,
+    every mu convention, and R-MAT."""
+    from label_check import labels_from_flags
+
+    rng = random.Random(5)
+    cases = []
+    for trial in range(80):
+        n = rng.randint(1, 120)
+        p = rng.choice([0.02, 0.1, 0.3, 0.6])
+        edges = [(u, v) for u in range(n) for v in range(u + 1, n) if rng.random() < p]
+        cases.append((bf.csr(n, edges), rng.choice([0.1, 0.3, 0.5, 0.7, 0.9]), rng.choice([-1, 0, 1, 2, 3, 5])))
+    g = sgc.HostGraph.rmat(13, 16, seed=9)
+    for eps, mu in [(0.3, 2), (0.5, 5), (0.2, 10), (0.6, 0)]:
+        cases.append(((np.array(g.row_ptr), np.array(g.adj)), eps, mu))
+    for (rp, adj), eps, mu in cases:
+        labels, core, sim, st = ob.orc_scan(rp, adj, eps, mu)
+        l2, c2, st2 = ob.orc_labels_from_sim(rp, adj, sim, mu)
+        assert np.array_equal(l2, labels) and np.array_equal(c2, core), (eps, mu)
+        for k in ("cores", "clusters", "borders", "hubs", "outliers", "similar_edges"):
+            assert st2[k] == st[k], (k, eps, mu)
+
+
+def test_labels_from_sim_on_arbitrary_flags():
+    """Flags no eps produces (random symmetric subsets): orc_labels_from_sim
+    against the independent vectorised rules of label_check."""
+    from label_check import labels_from_flags
+
+    g = sgc.HostGraph.rmat(12, 16, seed=3)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    n = len(rp) - 1
+    src = np.repeat(np.arange(n), np.diff(rp).astype(np.int64))
+    lo, hi = np.minimum(src, adj), np.maximum(src, adj)
+    for seed, frac, mu in [(1, 0.3, 2), (2, 0.6, 4), (3, 0.1, 1), (4, 0.5, 0)]:
+        key = lo.astype(np.int64) * n + hi  # the same coin for both entries of an edge
+        coin = (np.random.default_rng(seed).random(int(key.max()) + 1) < frac) if len(key) else np.zeros(0, bool)
+        sim = coin[key].astype(np.uint8)
+        labels, core, _ = ob.orc_labels_from_sim(rp, adj, sim, mu)
+        l2, c2 = labels_from_flags(rp, adj, sim, mu)
+        assert np.array_equal(core, c2) and np.array_equal(labels, l2), (seed, mu)
+
+
+def test_prep_counts_match_scan():
+    rng = random.Random(8)
+    for trial in range(20):
+        n = rng.randint(2, 200)
+        edges = [(rng.randrange(n), rng.randrange(n)) for _ in range(rng.randint(0, 2000))]
+        rp, adj = bf.csr(n, edges)
+        for eps in (0.1, 0.3, 0.5, 0.9):
+            _, _, _, st = ob.orc_scan(rp, adj, eps, 2)
+            assert ob.orc_prep_counts(rp, adj, eps) == (st["prep_n"], st["prep_y"], st["prep_u"])
