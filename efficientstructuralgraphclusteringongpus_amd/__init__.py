@@ -24,7 +24,7 @@ import numpy as np
 __all__ = [
     "SGCError", "HostGraph", "Context", "DeviceGraph", "ScanStats", "ScanResult",
     "scan", "scan_device", "scan_ooc", "ooc_min_budget", "OocStats", "Server", "QueryStats", "Tuning",
-    "copy_bench", "write_result", "read_result", "ROLE_NAMES", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
+    "copy_bench", "scan_device_noalloc", "noalloc_max_bytes", "noalloc_bytes", "write_result", "read_result", "ROLE_NAMES", "GPUScan", "library_path", "load_library", "HUB", "OUTLIER",
 ]
 
 HUB = -1
@@ -239,6 +239,9 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_scan_ooc", C.c_int,
          [_VP, _VP, C.c_double, C.c_int32, C.c_uint64, _VP, _VP, _VP, _P(_Stats), _P(_OocStats)])
     _sig(lib, "sgc_scan_device", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
+    _sig(lib, "sgc_scan_device_noalloc", C.c_int, [_VP, _VP, C.c_double, C.c_int32, _VP, _VP, _VP, _P(_Stats)])
+    _sig(lib, "sgc_noalloc_max_bytes", C.c_uint64, [])
+    _sig(lib, "sgc_noalloc_bytes", C.c_uint64, [_VP])
     _sig(lib, "sgc_GPUScan", _P(C.c_int32), [_P(_GraphT), _VP, _VP, C.c_double, C.c_int32])
     _sig(lib, "sgc_copy_bench", C.c_int, [_VP, C.c_uint64, _P(C.c_double)])
     _LIB = lib
@@ -521,6 +524,26 @@ def scan_device(ctx: Context, dg: DeviceGraph, eps: float, mu: int, d_labels: in
     _check(lib.sgc_scan_device(ctx.handle, dg.handle, float(eps), int(mu), d_labels, d_core, d_sim,
                                C.byref(st)))
     return ScanStats._from(st)
+
+
+def scan_device_noalloc(ctx: Context, dg: DeviceGraph, eps: float, mu: int, d_labels: int | None = None,
+                        d_core: int | None = None, d_sim: int | None = None) -> ScanStats:
+    """scan_device without any device allocation: the pass's work arrays live
+    in the library's static per-GPU scratch (noalloc_max_bytes() bytes);
+    raises SGCError (ERANGE) for a graph whose pass needs more."""
+    lib = load_library()
+    st = _Stats()
+    _check(lib.sgc_scan_device_noalloc(ctx.handle, dg.handle, float(eps), int(mu), d_labels, d_core, d_sim,
+                                       C.byref(st)))
+    return ScanStats._from(st)
+
+
+def noalloc_max_bytes() -> int:
+    return int(load_library().sgc_noalloc_max_bytes())
+
+
+def noalloc_bytes(dg: DeviceGraph) -> int:
+    return int(load_library().sgc_noalloc_bytes(dg.handle))
 
 
 def copy_bench(ctx: Context, nbytes: int) -> float:

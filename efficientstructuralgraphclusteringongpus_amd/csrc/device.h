@@ -67,6 +67,7 @@ struct sgc_ctx {
   uint64_t* host_counters = nullptr;  // pinned mirror
   hipEvent_t ev[8] = {};
   int sim_impl = sgc::kSimTiered;
+  void* scratch = nullptr;  // this GPU's static scratch (sgc_scan_device_noalloc)
 };
 
 namespace sgc {

@@ -14,7 +14,35 @@
 
 using namespace sgc;
 
+// Work memory of sgc_scan_device_noalloc: one copy per GPU, held from the
+// library's load on that GPU until the process ends.  Host code never uses
+// this name as an address: the device address of each GPU's copy is taken
+// once per device by hipGetSymbolAddress (scratch_ptr_for) and passed on.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device and kept.
+int scratch_ptr_for(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return fail(SGC_EINVAL, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SGC_EDEVICE, "address of the static scratch on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return SGC_OK;
+}
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -33,8 +61,13 @@ int ensure(DevBuf& b, size_t bytes) {
   return SGC_OK;
 }
 
-// Carves the per-pass arrays out of the context workspace.
-int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
+// Byte offsets of one pass's arrays for graph g inside a work buffer; `bytes`
+// is the buffer size they need.
+struct PassLayout {
+  size_t flag, sd, parent, label, lo, hi, core, cnt_u, work_u, uedge, units, bytes;
+};
+
+PassLayout pass_layout(const sgc_dgraph* g) {
   const size_t n = g->n, m = g->m;
   size_t off = 0;
   auto take = [&off](size_t bytes) {
@@ -42,34 +75,46 @@ int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
     off = align_up(off + std::max<size_t>(bytes, 1), 256);
     return o;
   };
-  const size_t o_flag = take(m);
-  const size_t o_sd = take(n * 4);
-  const size_t o_parent = take(n * 4);
-  const size_t o_label = take(n * 4);
-  const size_t o_lo = take(n * 4);
-  const size_t o_hi = take(n * 4);
-  const size_t o_core = take(n);
-  const size_t o_cnt_u = take(n * 4);
-  const size_t o_work_u = take(n * 8);
-  const size_t o_uedge = take(m * sizeof(uint4));
-  const size_t o_units = take(unit_capacity(m) * sizeof(uint4));
-  int rc = ensure(ctx->work, off);
-  if (rc) return rc;
-  char* base = (char*)ctx->work.ptr;
-  pb->flag = (uint8_t*)(base + o_flag);
-  pb->sd = (uint32_t*)(base + o_sd);
-  pb->parent = (int32_t*)(base + o_parent);
-  pb->label = (int32_t*)(base + o_label);
-  pb->lo = (int32_t*)(base + o_lo);
-  pb->hi = (int32_t*)(base + o_hi);
-  pb->core = (uint8_t*)(base + o_core);
-  pb->cnt_u = (uint32_t*)(base + o_cnt_u);
-  pb->work_u = (uint64_t*)(base + o_work_u);
-  pb->uedge = (uint4*)(base + o_uedge);
-  pb->units = (uint4*)(base + o_units);
+  PassLayout L;
+  L.flag = take(m);
+  L.sd = take(n * 4);
+  L.parent = take(n * 4);
+  L.label = take(n * 4);
+  L.lo = take(n * 4);
+  L.hi = take(n * 4);
+  L.core = take(n);
+  L.cnt_u = take(n * 4);
+  L.work_u = take(n * 8);
+  L.uedge = take(m * sizeof(uint4));
+  L.units = take(unit_capacity(m) * sizeof(uint4));
+  L.bytes = off;
+  return L;
+}
+
+void bind_pass(char* base, const PassLayout& L, const sgc_dgraph* g, uint64_t* cnt, PassBuffers* pb) {
+  pb->flag = (uint8_t*)(base + L.flag);
+  pb->sd = (uint32_t*)(base + L.sd);
+  pb->parent = (int32_t*)(base + L.parent);
+  pb->label = (int32_t*)(base + L.label);
+  pb->lo = (int32_t*)(base + L.lo);
+  pb->hi = (int32_t*)(base + L.hi);
+  pb->core = (uint8_t*)(base + L.core);
+  pb->cnt_u = (uint32_t*)(base + L.cnt_u);
+  pb->work_u = (uint64_t*)(base + L.work_u);
+  pb->uedge = (uint4*)(base + L.uedge);
+  pb->units = (uint4*)(base + L.units);
   pb->tiny = (uint32_t*)pb->label;
-  pb->ucap = (uint32_t)unit_capacity(m);
-  pb->cnt = (uint64_t*)ctx->counters.ptr;
+  pb->ucap = (uint32_t)unit_capacity(g->m);
+  pb->cnt = cnt;
+}
+
+// Carves the per-pass arrays out of the context workspace (grown by
+// hipMalloc when too small).
+int carve(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb) {
+  const PassLayout L = pass_layout(g);
+  int rc = ensure(ctx->work, L.bytes);
+  if (rc) return rc;
+  bind_pass((char*)ctx->work.ptr, L, g, (uint64_t*)ctx->counters.ptr, pb);
   return SGC_OK;
 }
 
@@ -91,8 +136,11 @@ extern "C" int sgc_ctx_create(int device, sgc_ctx** out) {
   SGC_HIP(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return fail(SGC_EINVAL, "device %d not present (%d devices)", device, count);
   SGC_HIP(hipSetDevice(device));
+  void* scratch_ptr = nullptr;
+  if (int rc = scratch_ptr_for(device, &scratch_ptr)) return rc;
   sgc_ctx* ctx = new sgc_ctx();
   ctx->device = device;
+  ctx->scratch = scratch_ptr;
   hipDeviceProp_t prop;
   SGC_HIP(hipGetDeviceProperties(&prop, device));
   strncpy(ctx->name, prop.name[0] ? prop.name : prop.gcnArchName, sizeof(ctx->name) - 1);
@@ -276,10 +324,9 @@ int check_unit_overflow(const uint64_t* c, uint64_t m) {
 
 namespace {
 
+// One plain pass over g with the arrays in *pb (carved by the caller).
 int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* pb, sgc_stats* st) {
   SGC_HIP(hipSetDevice(ctx->device));
-  int rc = carve(ctx, g, pb);
-  if (rc) return rc;
   const ScanParams p{eps, mu};
   hipStream_t s = ctx->stream;
   SGC_HIP(hipEventRecord(ctx->ev[0], s));
@@ -446,8 +493,10 @@ extern "C" int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t 
   clear_error();
   if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(SGC_EINVAL, "graph belongs to another context");
+  SGC_HIP(hipSetDevice(ctx->device));
   PassBuffers pb;
-  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  int rc = carve(ctx, g, &pb);
+  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   if (d_labels) SGC_HIP(hipMemcpyAsync(d_labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
@@ -462,13 +511,49 @@ extern "C" int sgc_scan(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int
   clear_error();
   if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(SGC_EINVAL, "graph belongs to another context");
+  SGC_HIP(hipSetDevice(ctx->device));
   PassBuffers pb;
-  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  int rc = carve(ctx, g, &pb);
+  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   if (labels) SGC_HIP(hipMemcpyAsync(labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
   if (core) SGC_HIP(hipMemcpyAsync(core, pb.core, g->n, hipMemcpyDeviceToHost, s));
   if (sim && g->m) SGC_HIP(hipMemcpyAsync(sim, pb.flag, g->m, hipMemcpyDeviceToHost, s));
+  SGC_HIP(hipStreamSynchronize(s));
+  return SGC_OK;
+}
+
+extern "C" uint64_t sgc_noalloc_max_bytes(void) { return kScratchBytes; }
+
+extern "C" uint64_t sgc_noalloc_bytes(const sgc_dgraph* g) { return g ? pass_layout(g).bytes : 0; }
+
+// sgc_scan_device with the pass's work arrays in the static scratch of the
+// context's GPU instead of the context workspace: no hipMalloc, hipFree or
+// allocating library call on any path.  Calls on one GPU share its scratch
+// (contexts included) and must not overlap; every array a pass reads before
+// writing is cleared by the pass itself (k_init).
+extern "C" int sgc_scan_device_noalloc(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int32_t* d_labels,
+                                       uint8_t* d_core, uint8_t* d_sim, sgc_stats* st) {
+  clear_error();
+  if (!ctx || !g) return fail(SGC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(SGC_EINVAL, "graph belongs to another context");
+  if (!ctx->scratch) return fail(SGC_EDEVICE, "no static scratch on device %d", ctx->device);
+  const PassLayout L = pass_layout(g);
+  if (L.bytes > kScratchBytes)
+    return fail(SGC_ERANGE,
+                "a pass over n=%u, m=%u needs %zu bytes of work memory, more than the %zu bytes of static scratch "
+                "behind sgc_scan_device_noalloc; use sgc_scan_device",
+                g->n, g->m, L.bytes, kScratchBytes);
+  SGC_HIP(hipSetDevice(ctx->device));
+  PassBuffers pb;
+  bind_pass((char*)ctx->scratch, L, g, (uint64_t*)ctx->counters.ptr, &pb);
+  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  if (d_labels) SGC_HIP(hipMemcpyAsync(d_labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
+  if (d_core) SGC_HIP(hipMemcpyAsync(d_core, pb.core, g->n, hipMemcpyDeviceToDevice, s));
+  if (d_sim && g->m) SGC_HIP(hipMemcpyAsync(d_sim, pb.flag, g->m, hipMemcpyDeviceToDevice, s));
   SGC_HIP(hipStreamSynchronize(s));
   return SGC_OK;
 }

@@ -233,6 +233,27 @@ int sgc_scan(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* labe
 int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* d_labels,
                     uint8_t* d_core, uint8_t* d_sim, sgc_stats* stats);
 
+/* Same pass as sgc_scan_device, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference, whose GPUScan allocates and frees all its
+ * device memory per call (GPUSCAN++/GPUScan.cu:800-1009).  The pass's work
+ * arrays live in a static device array of sgc_noalloc_max_bytes() bytes
+ * (64 MiB) that each GPU holds from the library's load until the process
+ * ends; the call makes no hipMalloc, hipFree or allocating library call.
+ * The caller passes the resident graph (uploaded beforehand) and the
+ * optional device outputs, as for sgc_scan_device.  SGC_ERANGE when the
+ * pass needs more than the scratch holds (sgc_noalloc_bytes(dg) bytes: about
+ * 25 B per entry and 33 B per vertex).  Every call on one GPU shares that
+ * GPU's scratch, whatever the context: calls that overlap on one GPU race;
+ * each pass clears what it reads before writing. */
+int sgc_scan_device_noalloc(sgc_ctx* ctx, sgc_dgraph* dg, double eps, int32_t mu, int32_t* d_labels,
+                            uint8_t* d_core, uint8_t* d_sim, sgc_stats* stats);
+/* Bytes of the static scratch behind sgc_scan_device_noalloc. */
+uint64_t sgc_noalloc_max_bytes(void);
+/* Work bytes one pass over dg needs (sgc_scan_device_noalloc accepts dg when
+ * this is at most sgc_noalloc_max_bytes()). */
+uint64_t sgc_noalloc_bytes(const sgc_dgraph* dg);
+
 /* Calibration aid with no counterpart in the reference: copies `bytes`
  * between two fresh device buffers with a plain streaming kernel (k_copy),
  * twice, and reports the second copy's time.  Its HBM byte counters
