@@ -404,6 +404,9 @@ def main() -> int:
                          "(value is the page-locked one when timed)")
     ap.add_argument("--serve-queries", type=int, default=2000,
                     help="queries of the serving run (32 resident graphs); 0 skips it")
+    ap.add_argument("--prune", action="store_true",
+                    help="core-check pruning on (the reference's skip of edges whose endpoints are both decided); "
+                         "results are the same either way")
     ap.add_argument("--copy-gib", type=float, default=1.0,
                     help="size of the plain device copy run first as the HBM counter yardstick; 0 skips it")
     args = ap.parse_args()
@@ -431,6 +434,8 @@ def main() -> int:
     work = {eps: g.similarity_work(eps) for eps in eps_list}
 
     ctx = sgc.Context(0)
+    if args.prune:
+        ctx.set_pruning(True)
     copy = None
     if args.copy_gib > 0:
         nbytes = int(args.copy_gib * (1 << 30))
@@ -461,7 +466,7 @@ def main() -> int:
             f"(workgroup tier {st.ms_similarity_large:.2f}), U={st.u_edges} streamed={st.probe_elements:.3g} "
             f"loaded={st.loaded_elements:.3g} claims={st.claims:.3g} "
             f"({st.probe_elements * 4 / max(st.ms_similarity, 1e-9) / 1e6:.0f} GB/s) cores={st.cores} "
-            f"clusters={st.clusters}")
+            f"clusters={st.clusters} pruned={st.pruned} deferred={st.deferred_edges} redo={st.redo_edges}")
 
     passes = len(stats)
     ms_per_step = wall / args.steps * 1e3
@@ -483,11 +488,15 @@ def main() -> int:
               for k in ("prep", "similarity", "similarity_large", "core", "cluster", "border", "classify", "total")}
     per_config = {}
     for eps, mu, st in stats:
-        d = per_config.setdefault(f"eps={eps} mu={mu}", {"pass": [], "similarity": [], "similarity_large": []})
+        d = per_config.setdefault(f"eps={eps} mu={mu}", {"pass": [], "similarity": [], "similarity_large": [],
+                                                         "deferred_edges": [], "redo_edges": []})
+        d["deferred_edges"].append(st.deferred_edges)
+        d["redo_edges"].append(st.redo_edges)
         d["pass"].append(st.ms_total)
         d["similarity"].append(st.ms_similarity)
         d["similarity_large"].append(st.ms_similarity_large)
-    per_config = {k: {f"{q}_ms": round(sum(v) / len(v), 3) for q, v in d.items()} for k, d in per_config.items()}
+    per_config = {k: {(q if q.endswith("edges") else f"{q}_ms"): round(sum(v) / len(v), 3) for q, v in d.items()}
+                  for k, d in per_config.items()}
 
     serving = None
     if args.serve_queries > 0:
@@ -519,6 +528,7 @@ def main() -> int:
             "n": g.n,
             "edges": E,
             "passes_per_step": len(params),
+            "core_check_pruning": bool(args.prune),
             "max_degree": g.max_degree,
             "device": ctx.device_name,
         },

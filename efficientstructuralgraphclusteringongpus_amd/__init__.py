@@ -60,6 +60,7 @@ class _Stats(C.Structure):
         ("hubs", C.c_uint64), ("outliers", C.c_uint64),
         ("alg_bytes", C.c_double), ("probe_elements", C.c_double),
         ("loaded_elements", C.c_double), ("claims", C.c_double),
+        ("deferred_edges", C.c_uint64), ("redo_edges", C.c_uint64), ("pruned", C.c_uint32),
     ]
 
 
@@ -149,6 +150,9 @@ class ScanStats:
     probe_elements: float
     loaded_elements: float
     claims: float
+    deferred_edges: int = 0
+    redo_edges: int = 0
+    pruned: int = 0
 
     @classmethod
     def _from(cls, s: _Stats) -> "ScanStats":
@@ -213,6 +217,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_ctx_sync", C.c_int, [_VP])
     _sig(lib, "sgc_ctx_device_name", C.c_char_p, [_VP])
     _sig(lib, "sgc_ctx_set_similarity_kernel", C.c_int, [_VP, C.c_int])
+    _sig(lib, "sgc_ctx_set_pruning", C.c_int, [_VP, C.c_int])
     _sig(lib, "sgc_dgraph_upload", C.c_int, [_VP, _VP, _P(_VP)])
     _sig(lib, "sgc_dgraph_free", None, [_VP])
     _sig(lib, "sgc_dgraph_n", C.c_uint64, [_VP])
@@ -432,6 +437,11 @@ class Context:
 
     def set_similarity_kernel(self, kind: int) -> None:
         _check(self._lib.sgc_ctx_set_similarity_kernel(self._h, int(kind)))
+
+    def set_pruning(self, enable: bool) -> None:
+        """Core-check pruning in passes that return no similarity flags
+        (default off; results are the same either way)."""
+        _check(self._lib.sgc_ctx_set_pruning(self._h, int(bool(enable))))
 
 
 class DeviceGraph:

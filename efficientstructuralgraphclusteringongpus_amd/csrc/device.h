@@ -42,8 +42,11 @@ enum Counter {
   kCntTiny,     // tiny owners listed by the planner for the tiny-owner kernel
   kCntOverflow, // owned undecided entries, when more than the unit array holds (asymmetric input)
   kCntMarked,   // cores marked by the core pass: none lets the later edge passes return at once
-  kCntTimer0 = 20,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
-  kCntSlots = 32
+  kCntRedo,     // deferred edges the union-find and border passes need: decided after all
+  kCntRedoOwned,  // owned undecided entries of that second similarity round
+  kCntDeferred,   // edges deferred by the core-check pruning and never needed
+  kCntTimer0 = 32,  // phase timers of debug builds (SGC_PHASE_TIMERS): 5 per tier
+  kCntSlots = 48
 };
 
 // Similarity kernels selectable per context.
@@ -68,6 +71,7 @@ struct sgc_ctx {
   hipEvent_t ev[8] = {};
   int sim_impl = sgc::kSimTiered;
   void* scratch = nullptr;  // this GPU's static scratch (sgc_scan_device_noalloc)
+  bool prune = false;       // core-check pruning in passes that return no similarity flags (sgc_ctx_set_pruning)
 };
 
 namespace sgc {
@@ -143,6 +147,11 @@ struct PassBuffers {
   // n: ids of the tiny owners (cnt[kCntTiny] of them).  Shares label's
   // memory, which is free until the core pass writes it.
   uint32_t* tiny;
+  // Pruned core check, until the core pass: similar (ny) and dissimilar (nn)
+  // edges seen per vertex, in lo's and hi's memory, and each vertex's
+  // state (kVUndecided / kVCore / kVNonCore) in core's.
+  uint32_t* ny;
+  uint32_t* nn;
   uint32_t ucap;
   uint64_t* cnt;     // kCntSlots
 };
@@ -150,7 +159,14 @@ struct PassBuffers {
 struct ScanParams {
   double eps;
   int32_t mu;
+  // core-check pruning (GPUSCAN++/GPUScan.cu:303): the workgroup tier defers
+  // an edge whose endpoints are both decided (core or not) when it starts
+  bool prune = false;
 };
+
+// Per-vertex state of the pruned core check, kept in PassBuffers::core until
+// the core pass writes the final flags there.
+enum : uint8_t { kVUndecided = 0, kVCore = 1, kVNonCore = 2 };
 
 // Carves the arrays of one pass over g from the context's work buffer.
 int carve_pass_buffers(sgc_ctx* ctx, const sgc_dgraph* g, PassBuffers* pb);
@@ -179,8 +195,17 @@ void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 // count_all: every edge is left to its owner (for exact common-neighbour
 // counts) instead of the degree-only decision.
+// With p.prune, also counts each row's degree-decided similar and dissimilar
+// entries (ny, nn).
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all = false);
-void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+// With p.prune, also sets every vertex's state from those counts.
+void launch_plan(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+// Second similarity round of a pruned pass (after the union-find pass): the
+// deferred edges the cluster and border passes need (core-core edges whose
+// roots differ, core-non-core edges) are marked undecided again, planned and
+// decided, and the union-find pass repeats.  Every kernel returns at once
+// when no deferred edge is needed.
+void launch_redo(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s);
 // `mid` (optional) is recorded between the workgroup-tier and wave-tier
 // kernels.  With cn_out the tiered kernels count every owned edge fully and
 // write |N(a) & N(b)| to both of its entries instead of flags.
@@ -198,7 +223,11 @@ void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s)
 // mu_dev (optional): read mu from device memory instead of p (captured launches)
 void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
                       const int32_t* mu_dev = nullptr);
-void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
+// gate >= 0: the kernel runs only when cnt[gate] is not 0
+void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s, int gate = -1);
+// marks the deferred edges a pruned pass needs after its union-find pass
+// (count in cnt[kCntRedo]); needs g->mirror
+void launch_redo_mark(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 // m_all: entries of the whole graph when g is one of several row chips
 void launch_classify_edges(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s, uint64_t m_all = 0);

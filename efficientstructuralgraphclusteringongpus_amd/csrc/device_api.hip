@@ -104,6 +104,8 @@ void bind_pass(char* base, const PassLayout& L, const sgc_dgraph* g, uint64_t* c
   pb->uedge = (uint4*)(base + L.uedge);
   pb->units = (uint4*)(base + L.units);
   pb->tiny = (uint32_t*)pb->label;
+  pb->ny = (uint32_t*)pb->lo;
+  pb->nn = (uint32_t*)pb->hi;
   pb->ucap = (uint32_t)unit_capacity(g->m);
   pb->cnt = cnt;
 }
@@ -186,6 +188,13 @@ extern "C" int sgc_ctx_set_similarity_kernel(sgc_ctx* ctx, int kind) {
   if (kind != SGC_SIM_KERNEL_TIERED && kind != SGC_SIM_KERNEL_WAVE_PER_EDGE)
     return fail(SGC_EINVAL, "unknown similarity kernel %d", kind);
   ctx->sim_impl = kind == SGC_SIM_KERNEL_TIERED ? kSimTiered : kSimWavePerEdge;
+  return SGC_OK;
+}
+
+extern "C" int sgc_ctx_set_pruning(sgc_ctx* ctx, int enable) {
+  clear_error();
+  if (!ctx) return fail(SGC_EINVAL, "null context");
+  ctx->prune = enable != 0;
   return SGC_OK;
 }
 
@@ -325,20 +334,30 @@ int check_unit_overflow(const uint64_t* c, uint64_t m) {
 namespace {
 
 // One plain pass over g with the arrays in *pb (carved by the caller).
-int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* pb, sgc_stats* st) {
+// want_flags: the caller takes the similarity flags, so every edge is
+// decided (no core-check pruning).
+int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* pb, sgc_stats* st,
+             bool want_flags) {
   SGC_HIP(hipSetDevice(ctx->device));
-  const ScanParams p{eps, mu};
+  // The reference's pruned core check (GPUSCAN++/GPUScan.cu:303) needs
+  // mu >= 1 (for mu <= 0 no vertex is ever decided not core) and the
+  // reverse-entry index of a resident graph.
+  const bool prune = ctx->prune && mu >= 1 && !want_flags && g->mirror && ctx->sim_impl == kSimTiered;
+  const ScanParams p{eps, mu, prune};
   hipStream_t s = ctx->stream;
   SGC_HIP(hipEventRecord(ctx->ev[0], s));
   launch_init(g, *pb, s);
   launch_prep(g, *pb, p, s);
-  launch_plan(g, *pb, s);
+  launch_plan(g, *pb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[1], s));
   launch_similarity(g, *pb, p, ctx->cu_count, ctx->sim_impl, ctx->ev[7], s);
   SGC_HIP(hipEventRecord(ctx->ev[2], s));
   launch_core(g, *pb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[3], s));
-  launch_cluster(g, *pb, s);
+  launch_union(g, *pb, s);
+  // deferred edges the clusters or borders need, decided now
+  if (p.prune) launch_redo(g, *pb, p, ctx->cu_count, s);
+  launch_flatten(g, *pb, s);
   SGC_HIP(hipEventRecord(ctx->ev[4], s));
   launch_border(g, *pb, s);
   SGC_HIP(hipEventRecord(ctx->ev[5], s));
@@ -370,6 +389,9 @@ int run_pass(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, PassBuffers* p
     st->probe_elements = (double)c[kCntProbes];
     st->loaded_elements = (double)c[kCntLoaded];
     st->claims = (double)c[kCntClaims];
+    st->redo_edges = c[kCntRedo];
+    st->deferred_edges = c[kCntDeferred] - c[kCntRedo];
+    st->pruned = prune ? 1 : 0;
 #ifdef SGC_PHASE_TIMERS
     fprintf(stderr, "phase ticks (unit, window, stream, barrier, final) large: %llu %llu %llu %llu %llu small: "
             "%llu %llu %llu %llu %llu\n", (unsigned long long)c[kCntTimer0], (unsigned long long)c[kCntTimer0 + 1],
@@ -395,7 +417,7 @@ int time_similarity(sgc_ctx* ctx, sgc_dgraph* g, double eps, PassBuffers* pb, do
   hipStream_t s = ctx->stream;
   launch_init(g, *pb, s);
   launch_prep(g, *pb, p, s);
-  launch_plan(g, *pb, s);
+  launch_plan(g, *pb, p, s);
   SGC_HIP(hipEventRecord(ctx->ev[1], s));
   launch_similarity(g, *pb, p, ctx->cu_count, kSimTiered, nullptr, s);
   SGC_HIP(hipEventRecord(ctx->ev[2], s));
@@ -496,7 +518,7 @@ extern "C" int sgc_scan_device(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t 
   SGC_HIP(hipSetDevice(ctx->device));
   PassBuffers pb;
   int rc = carve(ctx, g, &pb);
-  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st);
+  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st, d_sim != nullptr);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   if (d_labels) SGC_HIP(hipMemcpyAsync(d_labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));
@@ -514,7 +536,7 @@ extern "C" int sgc_scan(sgc_ctx* ctx, sgc_dgraph* g, double eps, int32_t mu, int
   SGC_HIP(hipSetDevice(ctx->device));
   PassBuffers pb;
   int rc = carve(ctx, g, &pb);
-  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st);
+  if (!rc) rc = run_pass(ctx, g, eps, mu, &pb, st, sim != nullptr);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   if (labels) SGC_HIP(hipMemcpyAsync(labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
@@ -548,7 +570,7 @@ extern "C" int sgc_scan_device_noalloc(sgc_ctx* ctx, sgc_dgraph* g, double eps, 
   SGC_HIP(hipSetDevice(ctx->device));
   PassBuffers pb;
   bind_pass((char*)ctx->scratch, L, g, (uint64_t*)ctx->counters.ptr, &pb);
-  int rc = run_pass(ctx, g, eps, mu, &pb, st);
+  int rc = run_pass(ctx, g, eps, mu, &pb, st, d_sim != nullptr);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   if (d_labels) SGC_HIP(hipMemcpyAsync(d_labels, pb.label, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s));

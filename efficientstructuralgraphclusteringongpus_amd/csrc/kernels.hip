@@ -108,14 +108,19 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_degrees(uint32_t m, const u
 // Per-pass initialisation
 // ---------------------------------------------------------------------------
 
+// lo / hi hold the pruned core check's similar / dissimilar counts and core
+// its vertex states until the core pass (k_core_mark) gives them their
+// final meaning.
 __global__ void __launch_bounds__(kBlock) k_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
-                                                 int32_t* __restrict__ hi, uint32_t* __restrict__ cnt_u,
+                                                 int32_t* __restrict__ hi, uint8_t* __restrict__ core,
+                                                 uint32_t* __restrict__ cnt_u,
                                                  uint64_t* __restrict__ work_u, uint64_t* __restrict__ cnt) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
     sd[x] = 0;
-    lo[x] = kNoLabel;
-    hi[x] = -1;
+    lo[x] = 0;
+    hi[x] = 0;
+    core[x] = kVUndecided;
     cnt_u[x] = 0;
     work_u[x] = 0;
   }
@@ -270,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t
                                                       const uint32_t* __restrict__ rb,
                                                       const uint32_t* __restrict__ re, uint8_t* __restrict__ core,
                                                       int32_t* __restrict__ parent, int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ lo, int32_t* __restrict__ hi,
                                                       uint64_t* __restrict__ cnt) {
   if (mu_dev) mu = *mu_dev;
   const uint32_t thr = mu >= 1 ? (uint32_t)mu : 1u;
@@ -280,6 +286,8 @@ __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t
     core[x] = c;
     parent[x] = (int32_t)x;
     label[x] = kNoLabel;
+    lo[x] = kNoLabel;  // classify's neighbour-label bounds
+    hi[x] = -1;
     if (c && rb) labeled += re[x] - rb[x];
     marked += c;
   }
@@ -312,11 +320,14 @@ __device__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
   }
 }
 
+// gate >= 0: runs only when cnt[gate] is not 0 (the union-find pass repeated
+// after the second similarity round of a pruned pass).
 __global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __restrict__ src,
                                                   const int32_t* __restrict__ adj, const uint8_t* __restrict__ flag,
                                                   const uint8_t* __restrict__ core, int32_t* parent,
-                                                  const uint64_t* __restrict__ cnt) {
+                                                  const uint64_t* __restrict__ cnt, int gate) {
   if (cnt[kCntMarked] == 0) return;  // no core: no core-core edge to join
+  if (gate >= 0 && cnt[gate] == 0) return;
   constexpr uint32_t C = kEdgeChunks;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
@@ -347,6 +358,51 @@ __global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* _
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride)
     if (core[x]) label[x] = uf_find(parent, (int32_t)x);
+}
+
+// Deferred edges of a pruned pass that the later passes need, after the
+// union-find pass over the known similar core-core edges: a core-core edge
+// whose endpoints are still in different trees (GPUSCAN++/GPUScan.cu:
+// 557-585 computes these on demand) and a core-non-core edge (a border
+// candidate, :692-719).  They are marked undecided again (owner side
+// kFlagUDown, the other kFlagUUp) for a second similarity round and counted
+// in cnt[kCntRedo].  A non-core-non-core edge is never needed: classify
+// reads labels only.  Edges deferred with both endpoints in one tree cannot
+// join anything.
+__global__ void __launch_bounds__(kBlock) k_redo_mark(uint32_t m, const uint32_t* __restrict__ src,
+                                                      const int32_t* __restrict__ adj,
+                                                      const uint32_t* __restrict__ mirror, uint8_t* flag,
+                                                      const uint8_t* __restrict__ core, int32_t* parent,
+                                                      uint64_t* __restrict__ cnt) {
+  if (cnt[kCntMarked] == 0 || cnt[kCntDeferred] == 0) return;
+  constexpr uint32_t C = kEdgeChunks;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  uint32_t redo = 0;
+  for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
+    bool d[C];
+    uint32_t b[C], a[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      const uint64_t k = k0 + j * stride;
+      d[j] = k < m && flag[k] == kFlagDeferDown;
+      b[j] = d[j] ? src[k] : 0u;
+      a[j] = d[j] ? (uint32_t)adj[k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      if (!d[j]) continue;
+      const bool ca = core[a[j]], cb = core[b[j]];
+      const bool need =
+          ca && cb ? uf_find(parent, (int32_t)a[j]) != uf_find(parent, (int32_t)b[j]) : ca != cb;
+      if (!need) continue;
+      const uint64_t k = k0 + j * stride;
+      flag[k] = kFlagUDown;
+      const uint32_t mk = mirror[k];  // 0xFFFFFFFF: no reverse entry (asymmetric input)
+      if (mk < m) flag[mk] = kFlagUUp;
+      ++redo;
+    }
+  }
+  block_atomic_add((unsigned long long*)&cnt[kCntRedo], redo);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,7 +598,7 @@ void launch_mirror(const sgc_dgraph* g, hipStream_t s) {
 }
 
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
-  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.cnt_u, b.work_u, b.cnt);
+  k_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, b.lo, b.hi, b.core, b.cnt_u, b.work_u, b.cnt);
 }
 
 void launch_similarity_wave(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
@@ -564,12 +620,18 @@ void launch_core_count(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s)
 void launch_core_mark(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s,
                       const int32_t* mu_dev) {
   k_core_mark<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.sd, p.mu, mu_dev, g->row_begin, g->row_end, b.core,
-                                                       b.parent, b.label, b.cnt);
+                                                       b.parent, b.label, b.lo, b.hi, b.cnt);
 }
 
-void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+void launch_union(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s, int gate) {
   if (g->m)
-    k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent, b.cnt);
+    k_union<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, b.flag, b.core, b.parent, b.cnt, gate);
+}
+
+void launch_redo_mark(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+  if (g->m && g->mirror)
+    k_redo_mark<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->src, g->adj, g->mirror, b.flag, b.core, b.parent,
+                                                         b.cnt);
 }
 
 void launch_flatten(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {

@@ -340,7 +340,7 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       const size_t k = chip_no;
       launch_init(&view, cb, s);
       launch_prep(&view, cb, p, s);
-      launch_plan(&view, cb, s);
+      launch_plan(&view, cb, p, s);
       SGC_HIP(hipEventRecord(R.t0[k & 1], s));
       launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[k & 1], s);
       SGC_HIP(hipEventRecord(R.t1[k & 1], s));

@@ -19,7 +19,18 @@ namespace sgc {
 // result.  After the similarity phase every flag is kFlagN or kFlagY.
 // kFlagUSkip: an owned undecided edge that an out-of-core chip leaves to the
 // chip holding both of its endpoints' blocks.
-enum : uint8_t { kFlagN = 0, kFlagY = 1, kFlagUUp = 2, kFlagUDown = 3, kFlagUSkip = 4 };
+// kFlagDeferUp / kFlagDeferDown: an undecided edge the pruned core check
+// never computed because both endpoints were decided (the rows as for
+// kFlagUUp / kFlagUDown); decided later only if a later pass needs it.
+enum : uint8_t {
+  kFlagN = 0,
+  kFlagY = 1,
+  kFlagUUp = 2,
+  kFlagUDown = 3,
+  kFlagUSkip = 4,
+  kFlagDeferUp = 5,
+  kFlagDeferDown = 6
+};
 
 // Degree-only pre-decision of the directed entry (u, v) with degrees du, dv
 // (GPUSCAN++/GPUScan.cu:143-169; orientation GPUSCAN++/Graph.cpp:75-83): the

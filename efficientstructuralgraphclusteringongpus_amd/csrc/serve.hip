@@ -418,7 +418,7 @@ extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uin
     hipStream_t s = ctx->stream;
     launch_init(G->dg, pb, s);
     launch_prep(G->dg, pb, p, s, true);
-    launch_plan(G->dg, pb, s);
+    launch_plan(G->dg, pb, p, s);
     launch_similarity(G->dg, pb, p, ctx->cu_count, kSimTiered, nullptr, s, G->cn);
     const hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess || hipGetLastError() != hipSuccess) {

@@ -73,6 +73,12 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 // 1. Degree-only pre-decision (GPUSCAN++/GPUScan.cu:133-172)
 // ---------------------------------------------------------------------------
 
+// ny/nn (pruned passes): each row's degree-decided similar / dissimilar
+// entries are added to its vertex's counts.  redo (the second round of a
+// pruned pass): the degree-only test is not rerun; the entries the round
+// marked kFlagUDown are planned as owned, their count goes to
+// cnt[kCntRedoOwned], and the kernel returns at once when cnt[kCntRedo] is 0.
+template <bool COUNTS, bool REDO>
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                  const uint32_t* __restrict__ src, const uint32_t* __restrict__ ndeg,
@@ -80,7 +86,10 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
                                                  double eps, bool count_all, PairFilter pf,
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
-                                                 uint4* __restrict__ uedge, uint64_t* __restrict__ cnt) {
+                                                 uint4* __restrict__ uedge, uint64_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ ny, uint32_t* __restrict__ nn) {
+  constexpr bool redo = REDO;
+  if (redo && cnt[kCntRedo] == 0) return;
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
@@ -111,19 +120,33 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       const uint64_t k = base + j * kWave + lane;
       const uint32_t row = u[j];
       bool owned = false;
+      uint32_t yn = 0;  // this entry's degree decision: 1 << 16 similar, 1 dissimilar
       if (k < m) {
         const uint32_t x = v[j];
         du[j] -= ub[j];
-        uint8_t f = prep_flag(row, x, du[j], dv[j], eps);
-        if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
-          f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
-        if (pf.on && f == kFlagUDown) {
-          const bool u0 = row - pf.lo0 < pf.hi0 - pf.lo0, u1 = row - pf.lo1 < pf.hi1 - pf.lo1;
-          const bool v0 = x - pf.lo0 < pf.hi0 - pf.lo0, v1 = x - pf.lo1 < pf.hi1 - pf.lo1;
-          if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
+        if (redo) {
+          owned = flag[k] == kFlagUDown;
+        } else {
+          uint8_t f = prep_flag(row, x, du[j], dv[j], eps);
+          if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
+            f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
+          if (pf.on && f == kFlagUDown) {
+            const bool u0 = row - pf.lo0 < pf.hi0 - pf.lo0, u1 = row - pf.lo1 < pf.hi1 - pf.lo1;
+            const bool v0 = x - pf.lo0 < pf.hi0 - pf.lo0, v1 = x - pf.lo1 < pf.hi1 - pf.lo1;
+            if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
+          }
+          flag[k] = f;
+          owned = f == kFlagUDown;
+          yn = f == kFlagY ? 0x10000u : (f == kFlagN ? 1u : 0u);
         }
-        flag[k] = f;
-        owned = f == kFlagUDown;
+      }
+      if (COUNTS && __ballot(yn != 0)) {
+        // the row's degree-decided entries, one pair of atomics per row segment
+        const uint32_t sum = wave_seg_reduce(row, yn, OpAdd());
+        if (wave_seg_head(row) && sum) {
+          if (sum >> 16) atomicAdd(&ny[row], sum >> 16);
+          if (sum & 0xFFFFu) atomicAdd(&nn[row], sum & 0xFFFFu);
+        }
       }
       om[j] = __ballot(owned);
       sb[j] = 0;
@@ -159,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
                                          mirror ? mirror[base + j * kWave + lane] : kNone, rb[v[j]], dv[j]);
     }
   }
-  block_atomic_add((unsigned long long*)&cnt[kCntUEdges], owned_total);
+  block_atomic_add((unsigned long long*)&cnt[redo ? kCntRedoOwned : kCntUEdges], owned_total);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,21 +205,36 @@ __device__ __forceinline__ uint32_t edges_per_unit(uint32_t cnt, uint64_t work, 
   return e ? e : 1u;
 }
 
+// vstate (pruned passes, first round): every vertex's state from its
+// degree-decided counts: core once ny >= mu, not core once d - nn < mu
+// (GPUSCAN++/GPUScan.cu:140-169).  redo: the second round of a pruned pass
+// (owned entries counted in cnt[kCntRedoOwned]; nothing to do when
+// cnt[kCntRedo] is 0).
 __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, SimTuning tune,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
                                                  uint4* __restrict__ units, uint32_t ucap,
-                                                 uint32_t* __restrict__ tiny, uint64_t* __restrict__ cnt) {
+                                                 uint32_t* __restrict__ tiny, uint64_t* __restrict__ cnt,
+                                                 uint8_t* __restrict__ vstate, const uint32_t* __restrict__ ny,
+                                                 const uint32_t* __restrict__ nn, int32_t mu, bool redo) {
   constexpr uint32_t V = 4;  // vertices per thread per block iteration
+  if (redo && cnt[kCntRedo] == 0) return;
   // Every unit holds at least one owned entry, so ucap units always suffice
   // when each undecided edge is owned once.  An input without a reverse
   // entry for every entry can own more: then no unit is written (the
   // regions from both ends would meet), and the host reports the count.
-  const uint64_t owned_all = cnt[kCntUEdges];
+  const uint64_t owned_all = cnt[redo ? kCntRedoOwned : kCntUEdges];
   if (owned_all > ucap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCntOverflow] = owned_all;
     return;
+  }
+  if (vstate) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+      const uint32_t d = re[x] - rb[x], y = ny[x], no = nn[x];
+      vstate[x] = y >= (uint32_t)mu ? kVCore : (d - no < (uint32_t)mu ? kVNonCore : kVUndecided);
+    }
   }
   unsigned long long* const ctr[3] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2],
                                       (unsigned long long*)&cnt[kCntTiny]};
@@ -637,7 +675,8 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
                 const uint32_t* __restrict__ mirror, double eps, uint8_t* __restrict__ flag,
                 uint32_t* __restrict__ cn_out, const uint32_t* __restrict__ woff, uint32_t wstride,
                 const uint4* __restrict__ uedge, const uint4* __restrict__ units, uint32_t ucap,
-                uint64_t* __restrict__ cnt, int units_slot, int queue_slot, bool hash_tables) {
+                uint64_t* __restrict__ cnt, int units_slot, int queue_slot, bool hash_tables,
+                uint8_t* __restrict__ vstate, uint32_t* __restrict__ ny, uint32_t* __restrict__ nn, int32_t mu) {
   __shared__ __align__(16) uint32_t tab[T::kTableWords];
   __shared__ unsigned long long e_state[T::kMaxEdges];  // hits << 32 | consumed
   // (c*, d(a), claim cap, slice end) of each edge: what a wave needs to size
@@ -647,6 +686,10 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
   // pass's slice may start (the row start before the first pass)
   __shared__ uint32_t e_ab[T::kMaxEdges], e_pos[T::kMaxEdges], e_mir[T::kMaxEdges], e_a[T::kMaxEdges];
   __shared__ uint32_t s_unit, s_cursor, s_kend, s_fail;
+  // pruned core check: the owner's similar / dissimilar edges decided in the
+  // unit, and the unit's deferred edges
+  __shared__ uint32_t s_by, s_bn, s_def;
+  uint64_t deferred = 0;  // thread 0: deferred edges of this workgroup
   __shared__ uint4 s_next;  // the next unit's descriptor
   // the owner row's window boundaries, from the window-offset index
   constexpr uint32_t kMaxOwnerWindows = 64;
@@ -709,17 +752,30 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       }
     }
     uint32_t my_k = kNone;  // this thread's edge entry (threads < ne), for the final writes
+    // pruned core check (GPUSCAN++/GPUScan.cu:303): an edge whose endpoints
+    // are both decided when the unit starts is deferred, not computed
+    uint32_t my_a = 0;
+    uint8_t my_sa = kVUndecided, sb = kVUndecided;
+    bool my_defer = false;
+    if (vstate && threadIdx.x == 0) s_by = s_bn = s_def = 0;
     if (threadIdx.x < ne) {
       const uint32_t t = threadIdx.x;
       const uint4 ed = uedge[un.x + t];  // (entry, reverse entry, row start of a, d(a))
       const uint32_t k = ed.x;
       uint32_t ab = 0, s = 0, cstar = 0, a = 0;
       if (k < m) {
-        if (T::kBitmap && woff) a = (uint32_t)adj[k];  // for the window-offset index
+        if (T::kBitmap && (woff || vstate)) a = (uint32_t)adj[k];  // window-offset index, vertex state
         ab = ed.z;
         s = ed.w;
+        if (vstate) {
+          my_a = a;
+          my_sa = vstate[a];
+          sb = vstate[b];
+          my_defer = my_sa != kVUndecided && sb != kVUndecided;
+        }
         const double rhs = predicate_rhs(eps, s, l);
         cstar = s <= kMaxThresholdRow && !cn_out ? min_common_threshold(rhs, s) : kNone;
+        if (my_defer) s = cstar = 0;  // nothing to stream: decided at once, not written as a result
       }
       my_k = k;
       e_a[t] = a;
@@ -979,14 +1035,43 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
         if (cn_out) {  // exact counts instead of flags
           cn_out[k] = hits;
           if (mpos != kNone) cn_out[mpos] = hits;
+        } else if (my_defer) {
+          flag[k] = kFlagDeferDown;
+          if (mpos != kNone) flag[mpos] = kFlagDeferUp;
+          atomicAdd(&s_def, 1u);
         } else {
           const uint8_t f = sim ? kFlagY : kFlagN;
           flag[k] = f;
           if (mpos != kNone) flag[mpos] = f;
+          if (vstate) {
+            // a's counts now; b's once per unit, below.  A vertex is core
+            // once it has mu similar edges and not core once fewer than mu
+            // edges are left that are not dissimilar (d(a) = s).
+            if (my_sa == kVUndecided) {
+              if (sim) {
+                if (atomicAdd(&ny[my_a], 1u) + 1u == (uint32_t)mu) vstate[my_a] = kVCore;
+              } else {
+                if (s - atomicAdd(&nn[my_a], 1u) == (uint32_t)mu) vstate[my_a] = kVNonCore;
+              }
+            }
+            atomicAdd(sim ? &s_by : &s_bn, 1u);
+          }
         }
       }
     }
     __syncthreads();
+    if (vstate && threadIdx.x == 0) deferred += s_def;
+    if (vstate && threadIdx.x == 0 && ne && sb == kVUndecided) {
+      const uint32_t y = s_by, no = s_bn, um = (uint32_t)mu;
+      if (y) {
+        const uint32_t old = atomicAdd(&ny[b], y);
+        if (old < um && old + y >= um) vstate[b] = kVCore;
+      }
+      if (no) {
+        const uint32_t old = atomicAdd(&nn[b], no);
+        if (l - old >= um && l - old - no < um) vstate[b] = kVNonCore;
+      }
+    }
     SGC_TICK(4);
   }
 #ifdef SGC_PHASE_TIMERS
@@ -995,6 +1080,7 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       atomicAdd((unsigned long long*)&cnt[kCntTimer0 + (T::kBitmap ? 0 : 5) + i], (unsigned long long)tm[i]);
 #endif
 #undef SGC_TICK
+  if (threadIdx.x == 0 && deferred) atomicAdd((unsigned long long*)&cnt[kCntDeferred], (unsigned long long)deferred);
   if (lane == 0 && claims) {
     atomicAdd((unsigned long long*)&cnt[kCntProbes], (unsigned long long)streamed);
     atomicAdd((unsigned long long*)&cnt[kCntLoaded], (unsigned long long)loaded);
@@ -1043,18 +1129,24 @@ void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
 
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
-  k_prep<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror,
-                                                   p.eps,
-                                                   count_all,
-                                                   g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt);
+  const uint32_t grid = grid_for(g->m, kBlock);
+  if (p.prune && !count_all)
+    k_prep<true, false><<<grid, kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror,
+                                                p.eps, count_all, g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt,
+                                                b.ny, b.nn);
+  else
+    k_prep<false, false><<<grid, kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror,
+                                                 p.eps, count_all, g->pair, b.flag, b.cnt_u, b.work_u, b.uedge, b.cnt,
+                                                 nullptr, nullptr);
 }
 
-void launch_plan(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
+void launch_plan(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
   if (g->m == 0) return;
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
   k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
-                                                   b.ucap, b.tiny, b.cnt);
+                                                   b.ucap, b.tiny, b.cnt, p.prune ? b.core : nullptr, b.ny, b.nn,
+                                                   p.mu, false);
 }
 
 void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl, hipEvent_t mid,
@@ -1066,14 +1158,19 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     return;
   }
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
+  // the core-check pruning runs in the workgroup tier, which launches first
+  // and holds the long rows; the later tiers decide every edge they own
+  const bool prune = p.prune && !cn_out && g->mirror;
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
                                                              cn_out, g->woff, g->wcount + 1, b.uedge, b.units,
                                                              b.ucap, b.cnt, kCntUnits2, kCntQueue2,
-                                                             g->tuning.hash_tables != 0);
+                                                             g->tuning.hash_tables != 0, prune ? b.core : nullptr,
+                                                             b.ny, b.nn, p.mu);
   if (mid) (void)hipEventRecord(mid, s);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
                                                                  b.flag, cn_out, nullptr, 0, b.uedge, b.units, b.ucap,
-                                                                 b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0);
+                                                                 b.cnt, kCntUnits1, kCntQueue1, g->tuning.hash_tables != 0,
+                                                                 nullptr, nullptr, nullptr, 0);
   if (g->tuning.tiny_work) {
     // the list's length is on the device: enough threads for every owner of
     // the chip's vertices, grid-striding beyond two per CU slot
@@ -1082,6 +1179,54 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
                                                                      g->mirror, p.eps, b.cnt_u, b.tiny, b.uedge,
                                                                      b.flag, cn_out, b.cnt);
   }
+}
+
+namespace {
+// Planner state for the second similarity round of a pruned pass: unit and
+// queue counters always (so that the round's kernels find no unit when it
+// has nothing to do), per-owner counts only when it has edges.
+__global__ void __launch_bounds__(kBlock) k_redo_init(uint32_t n, uint32_t* __restrict__ cnt_u,
+                                                      uint64_t* __restrict__ work_u, uint64_t* __restrict__ cnt) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[kCntUnits1] = cnt[kCntUnits2] = 0;
+    cnt[kCntQueue1] = cnt[kCntQueue2] = 0;
+    cnt[kCntTiny] = 0;
+    cnt[kCntRedoOwned] = 0;
+  }
+  if (cnt[kCntRedo] == 0) return;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    cnt_u[x] = 0;
+    work_u[x] = 0;
+  }
+}
+}  // namespace
+
+void launch_redo(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, hipStream_t s) {
+  if (g->m == 0 || !g->mirror) return;
+  launch_redo_mark(g, b, s);
+  k_redo_init<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, b.cnt_u, b.work_u, b.cnt);
+  k_prep<false, true><<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj, g->src,
+                                                                g->ndeg, g->mirror, p.eps, false, g->pair, b.flag,
+                                                                b.cnt_u, b.work_u, b.uedge, b.cnt, nullptr, nullptr);
+  // no tiny-owner list this round: it would live in the label array, which
+  // the core pass has initialised by now
+  SimTuning t = g->tuning;
+  if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
+  t.tiny_work = 0;
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
+                                                   b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, true);
+  const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
+  k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
+                                                             nullptr, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                             b.ucap, b.cnt, kCntUnits2, kCntQueue2,
+                                                             g->tuning.hash_tables != 0, nullptr, nullptr, nullptr, 0);
+  k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
+                                                                 b.flag, nullptr, nullptr, 0, b.uedge, b.units,
+                                                                 b.ucap, b.cnt, kCntUnits1, kCntQueue1,
+                                                                 g->tuning.hash_tables != 0, nullptr, nullptr, nullptr,
+                                                                 0);
+  launch_union(g, b, s, kCntRedo);
 }
 
 }  // namespace sgc

@@ -154,6 +154,15 @@ const char* sgc_ctx_device_name(const sgc_ctx* ctx);
 #define SGC_SIM_KERNEL_WAVE_PER_EDGE 1
 int sgc_ctx_set_similarity_kernel(sgc_ctx* ctx, int kind);
 
+/* Core-check pruning for passes that return no similarity flags (default
+ * off): the reference's skip of edges whose endpoints are both decided
+ * (GPUSCAN++/GPUScan.cu:303), with the edges the cluster and border passes
+ * need decided on demand (:557-585, :692-719).  Passes that return flags,
+ * passes with mu < 1 and the one-wave-per-edge kernel decide every edge.
+ * Results do not depend on it.  Off by default because on R-MAT it defers
+ * too few edges to pay for its bookkeeping (about 4 % slower at scale 24). */
+int sgc_ctx_set_pruning(sgc_ctx* ctx, int enable);
+
 typedef struct sgc_dgraph sgc_dgraph;
 
 /* Copies a host CSR into HBM (uint32 offsets; requires m < 2^32, else
@@ -216,6 +225,16 @@ typedef struct {
   double probe_elements; /* list elements consumed by them */
   double loaded_elements; /* list elements loaded by them (consumed or read past a window's end) */
   double claims;          /* wave claims of the tiered kernels */
+  /* Core-check pruning (GPUSCAN++/GPUScan.cu:303): in a pruned pass the
+   * workgroup-tier similarity kernel defers an edge whose endpoints are both
+   * already decided (core, or not core); after the union-find pass the
+   * deferred edges the clusters or borders need are decided after all
+   * (redo_edges), the rest are never decided (deferred_edges) and are not
+   * counted in similar_edges.  Labels and core flags are the same either
+   * way. */
+  uint64_t deferred_edges;
+  uint64_t redo_edges;
+  uint32_t pruned;        /* 1 if the pass ran pruned */
 } sgc_stats;
 
 /* Replaces GPUScan(graph_t*, int*, int*, double, int) (GPUSCAN++/GPUScan.h:7,

@@ -7,6 +7,8 @@ checked in two independent parts:
   the reference's predicate (short row longer than 46338, GPUSCAN++/
   GPUScan.cu:220-221) plus a uniform 0.5 % sample of all entries, against
   the oracle's per-edge similarity (prep + merge count + predicate);
+- the same pass without similarity flags (which runs the reference's pruned
+  core check, GPUScan.cu:303) gives the same labels and core flags;
 - labels and core flags: the oracle's phases after similarity
   (orc_labels_from_sim: core check, union-find with minimum-id roots,
   canonical borders, hubs/outliers, GPUScan.cu:276-798) run over the device's
@@ -66,6 +68,15 @@ def check_grid(ctx, g, grid, seed):
                 assert len(diff) == 0, f"labels differ at {diff[:10]}"
                 for k in ("cores", "clusters", "borders", "hubs", "outliers", "similar_edges"):
                     assert getattr(r.stats, k) == st[k], k
+                # the same pass without flags, with the pruned core check
+                ctx.set_pruning(True)
+                try:
+                    p = sgc.scan(ctx, dg, eps, mu)
+                finally:
+                    ctx.set_pruning(False)
+                assert p.stats.pruned == 1
+                assert np.array_equal(p.core, r.core) and np.array_equal(p.labels, r.labels), "pruned pass differs"
+                say(f"  pruned: {p.stats.deferred_edges} edges never decided, {p.stats.redo_edges} decided on demand")
                 out[(eps, mu)] = st
     finally:
         dg.close()

@@ -19,7 +19,9 @@ def say(*a):
 
 
 def one_shot(ctx, dg, eps, mu):
-    return sgc.scan(ctx, dg, eps, mu)
+    """A plain pass that decides every edge (flags requested), so that its
+    similar-edge count is complete; a pruned pass counts decided edges only."""
+    return sgc.scan(ctx, dg, eps, mu, want_sim=True)
 
 
 def drain(srv, tickets):
