@@ -161,7 +161,7 @@ int validate_graph(const sgc_host_graph* g, bool check_symmetry) {
 }
 
 int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint32_t* dst,
-                     sgc_host_graph** out) {
+                     sgc_host_graph** out, uint32_t** release) {
   *out = nullptr;
   if (n == 0 || n > (1ull << 31)) return fail(SGC_EINVAL, "vertex count %llu outside [1, 2^31]", (unsigned long long)n);
   std::vector<uint64_t> cnt(n + 1, 0);
@@ -196,6 +196,14 @@ int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint
     buf[__atomic_fetch_add(&fill[v], 1, __ATOMIC_RELAXED)] = u;
   }
   progress("graph build: entries scattered (%llu)", (unsigned long long)total);
+  // the pairs are no longer read: a caller that handed them over gets them
+  // freed before the final adjacency is allocated (peak host memory: pairs +
+  // scatter buffer, then scatter buffer + adjacency)
+  if (release) {
+    free(release[0]);
+    free(release[1]);
+    release[0] = release[1] = nullptr;
+  }
   std::vector<uint64_t> deg(n + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (uint64_t u = 0; u < n; ++u) {
@@ -451,9 +459,10 @@ extern "C" int sgc_graph_rmat(int scale, double edge_factor, double a, double b,
 #pragma omp parallel for schedule(static, 65536)
   for (uint64_t i = 0; i < draws; ++i) rmat_draw(i, scale, seed, t, &src[i], &dst[i]);
   progress("rmat scale %d: %llu draws", scale, (unsigned long long)draws);
-  int rc = build_from_pairs(n, draws, src, dst, out);
-  free(src);
-  free(dst);
+  uint32_t* pairs[2] = {src, dst};
+  int rc = build_from_pairs(n, draws, src, dst, out, pairs);
+  free(pairs[0]);  // null once build_from_pairs has released them
+  free(pairs[1]);
   return rc;
 }
 

@@ -21,8 +21,10 @@ sgc_host_graph* new_host_graph(uint64_t n, uint64_t m, bool pinned);
 // if requested, every (u,v) has its (v,u).
 int validate_graph(const sgc_host_graph* g, bool check_symmetry);
 // Builds a simple undirected graph from `count` directed draws: self loops
-// dropped, both directions added, duplicates removed, rows sorted.
+// dropped, both directions added, duplicates removed, rows sorted.  With
+// `release` (the malloc'd src and dst), both are freed, and release[0..1]
+// set to null, as soon as they are no longer read.
 int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint32_t* dst,
-                     sgc_host_graph** out);
+                     sgc_host_graph** out, uint32_t** release = nullptr);
 
 }  // namespace sgc

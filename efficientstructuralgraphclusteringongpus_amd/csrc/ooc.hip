@@ -43,12 +43,15 @@ __global__ void __launch_bounds__(kBlock) k_rows_absent(uint32_t n, const uint32
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_rows_range(uint32_t r0, uint32_t nr, const uint32_t* __restrict__ off,
-                                                       uint32_t* __restrict__ rb, uint32_t* __restrict__ re) {
+// Rows r0 .. r0 + nr of a chip: off[] holds their offsets from the block's
+// first entry, `shift` the block's first entry in the chip.
+__global__ void __launch_bounds__(kBlock) k_rows_range(uint32_t r0, uint32_t nr, uint32_t shift,
+                                                       const uint32_t* __restrict__ off, uint32_t* __restrict__ rb,
+                                                       uint32_t* __restrict__ re) {
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < nr; i += stride) {
-    rb[r0 + i] = off[i];
-    re[r0 + i] = off[i + 1];
+    rb[r0 + i] = shift + off[i];
+    re[r0 + i] = shift + off[i + 1];
   }
 }
 
@@ -78,21 +81,29 @@ struct Layout {
   }
 };
 
-// Bytes per entry of the two chip kinds (adj, src, flag, uedge record, work
-// units at one per two entries / adj, src, flag).
-constexpr uint64_t kSimChipBytesPerEntry = 4 + 4 + 1 + 16 + 8;
+// Similarity chips are two row blocks of at most C entries.  Their device
+// memory, per block entry: adj and src as [ j-buffer 0 | block i | j-buffer 1 ]
+// (3 x 4 B each; the second block of a chip sits right before the first in
+// buffer 0 or right after it in buffer 1, so every chip is one contiguous
+// entry range and the next chip's second block uploads while this one runs),
+// two flag arrays of 2C (consecutive chips alternate, so one chip's flags go
+// back to the host while the next one computes), the edge records (2C x 16 B)
+// and work units (C x 16 B) of one chip.  Row chips of the later phases hold
+// adj, src and flag.
+constexpr uint64_t kSimBytesPerBlockEntry = 3 * 4 + 3 * 4 + 2 * 2 + 2 * 16 + 16;
 constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
 struct ResidentLayout {
-  size_t deg, rb, re, off_i, off_j, sd, par, lab, lo, hi, core, cntu, work, cnt, bytes;
+  size_t deg, rb, re, off_i, off_j0, off_j1, sd, par, lab, lo, hi, core, cntu, work, cnt, bytes;
 };
 
 ResidentLayout resident_layout(uint32_t n) {
   Layout L;
   ResidentLayout r;
   r.deg = L.take((size_t)n * 4), r.rb = L.take((size_t)n * 4), r.re = L.take((size_t)n * 4);
-  r.off_i = L.take(((size_t)n + 1) * 4), r.off_j = L.take(((size_t)n + 1) * 4);
+  r.off_i = L.take(((size_t)n + 1) * 4), r.off_j0 = L.take(((size_t)n + 1) * 4);
+  r.off_j1 = L.take(((size_t)n + 1) * 4);
   r.sd = L.take((size_t)n * 4), r.par = L.take((size_t)n * 4), r.lab = L.take((size_t)n * 4);
   r.lo = L.take((size_t)n * 4), r.hi = L.take((size_t)n * 4), r.core = L.take(n);
   r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.cnt = L.take(kCntSlots * 8);
@@ -101,25 +112,28 @@ ResidentLayout resident_layout(uint32_t n) {
 }
 
 struct SimChipArrays {
-  int32_t* adj;
-  uint32_t* src;
-  uint8_t* flag;
-  uint4* uedge;
-  uint4* units;
+  int32_t* adj;      // 3C
+  uint32_t* src;     // 3C
+  uint8_t* flag[2];  // 2C each
+  uint4* uedge;      // 2C
+  uint4* units;      // unit_capacity(2C)
 };
 
-SimChipArrays sim_arrays(char* base, uint64_t cap, size_t* bytes) {
+SimChipArrays sim_arrays(char* base, uint64_t C, size_t* bytes) {
   Layout L;
-  const size_t o_adj = L.take(cap * 4), o_src = L.take(cap * 4), o_flag = L.take(cap),
-               o_uedge = L.take(cap * sizeof(uint4)), o_units = L.take(unit_capacity(cap) * sizeof(uint4));
+  const size_t o_adj = L.take(3 * C * 4), o_src = L.take(3 * C * 4), o_f0 = L.take(2 * C), o_f1 = L.take(2 * C),
+               o_uedge = L.take(2 * C * sizeof(uint4)), o_units = L.take(unit_capacity(2 * C) * sizeof(uint4));
   if (bytes) *bytes = L.bytes;
-  return SimChipArrays{(int32_t*)(base + o_adj), (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
-                       (uint4*)(base + o_uedge), (uint4*)(base + o_units)};
+  return SimChipArrays{(int32_t*)(base + o_adj),
+                       (uint32_t*)(base + o_src),
+                       {(uint8_t*)(base + o_f0), (uint8_t*)(base + o_f1)},
+                       (uint4*)(base + o_uedge),
+                       (uint4*)(base + o_units)};
 }
 
-size_t sim_chip_bytes(uint64_t cap) {
+size_t sim_chip_bytes(uint64_t C) {
   size_t b = 0;
-  sim_arrays(nullptr, cap, &b);
+  sim_arrays(nullptr, C, &b);
   return b;
 }
 
@@ -142,28 +156,40 @@ struct OocRun {
   const sgc_host_graph* g = nullptr;
   uint32_t n = 0;
   void* dev = nullptr;
-  uint32_t *deg = nullptr, *rb = nullptr, *re = nullptr, *off_i = nullptr, *off_j = nullptr;
+  uint32_t *deg = nullptr, *rb = nullptr, *re = nullptr, *off_i = nullptr, *off_j[2] = {nullptr, nullptr};
   PassBuffers pb{};
   char* chip = nullptr;
-  Staging stage[3];  // block i, and two for the j blocks
+  Staging stage[3];  // block i, and one per j buffer
+  // Side streams of the similarity phase (default priority): uploads and
+  // flag read-back run there, overlapped with the chips' kernels on the
+  // context stream.
+  hipStream_t h2d = nullptr, d2h = nullptr;
+  hipEvent_t up_i = nullptr, up_j[2] = {nullptr, nullptr};  // block uploads done
+  hipEvent_t comp[2] = {nullptr, nullptr};                  // the chip of buffer b done on the device
+  hipEvent_t back[2] = {nullptr, nullptr};                  // its flags back on the host
   uint8_t* h_flag[2] = {nullptr, nullptr};
   uint64_t* h_cnt[2] = {nullptr, nullptr};
-  hipEvent_t done[2] = {nullptr, nullptr}, t0[2] = {nullptr, nullptr}, tm[2] = {nullptr, nullptr},
-             t1[2] = {nullptr, nullptr};
+  hipEvent_t t0[2] = {nullptr, nullptr}, tm[2] = {nullptr, nullptr}, t1[2] = {nullptr, nullptr};
   uint8_t* flag = nullptr;  // whole-graph flags, pinned
   double bytes_h2d = 0, bytes_d2h = 0, ms_host = 0;
 
   ~OocRun() {
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t st : {h2d, d2h})
+      if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+      }
     for (Staging& st : stage) {
       if (st.adj) (void)hipHostFree(st.adj);
       if (st.off) (void)hipHostFree(st.off);
       if (st.free) (void)hipEventDestroy(st.free);
     }
+    if (up_i) (void)hipEventDestroy(up_i);
     for (int i = 0; i < 2; ++i) {
       if (h_flag[i]) (void)hipHostFree(h_flag[i]);
       if (h_cnt[i]) (void)hipHostFree(h_cnt[i]);
-      for (hipEvent_t e : {done[i], t0[i], tm[i], t1[i]})
+      for (hipEvent_t e : {up_j[i], comp[i], back[i], t0[i], tm[i], t1[i]})
         if (e) (void)hipEventDestroy(e);
     }
     if (flag) (void)hipHostFree(flag);
@@ -172,9 +198,11 @@ struct OocRun {
 };
 
 // One chip of the similarity phase: blocks i and j (j == i: one block).
+// j_first: the chip's entries hold block j first (j-buffer 0), else block i.
 struct ChipDesc {
   uint64_t r0i, r1i, r0j, r1j;
   uint64_t mi, mj;
+  bool j_first;
 };
 
 int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels, uint8_t* core_out, uint8_t* sim_out,
@@ -192,19 +220,20 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     return fail(SGC_ENOMEM, "device budget %llu bytes is below the %llu resident bytes the graph's %u vertices need",
                 (unsigned long long)budget, (unsigned long long)resident, n);
   const uint64_t room = budget - resident - 8192;
-  const uint64_t cap_sim = std::min<uint64_t>(room / kSimChipBytesPerEntry, kMaxChipEntries);
+  const uint64_t cap_block = std::min<uint64_t>(room / kSimBytesPerBlockEntry, kMaxChipEntries / 2);
   const uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries);
   double t0 = now_ms();
   std::vector<uint32_t> deg(n);
   ooc_degrees(g, deg.data());
   std::vector<uint64_t> blocks, row_bounds;
   uint64_t block_max = 0, row_max = 0;
-  int rc = ooc_plan_rows(g, cap_sim / 2, &blocks, &block_max);
+  int rc = ooc_plan_rows(g, cap_block, &blocks, &block_max);
   if (rc) return rc;
   rc = ooc_plan_rows(g, cap_row, &row_bounds, &row_max);
   if (rc) return rc;
-  const uint64_t sim_entries = std::max<uint64_t>(2 * block_max, 1);
-  const size_t chip_need = std::max(sim_chip_bytes(sim_entries), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
+  const uint64_t C = std::max<uint64_t>(block_max, 1);  // entries per block at most
+  const uint64_t sim_entries = 2 * C;                   // entries per chip at most
+  const size_t chip_need = std::max(sim_chip_bytes(C), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
   R.ms_host += now_ms() - t0;
 
   // ---- device and pinned host memory
@@ -217,7 +246,8 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   R.rb = (uint32_t*)(base + RLay.rb);
   R.re = (uint32_t*)(base + RLay.re);
   R.off_i = (uint32_t*)(base + RLay.off_i);
-  R.off_j = (uint32_t*)(base + RLay.off_j);
+  R.off_j[0] = (uint32_t*)(base + RLay.off_j0);
+  R.off_j[1] = (uint32_t*)(base + RLay.off_j1);
   PassBuffers& pb = R.pb;
   pb.sd = (uint32_t*)(base + RLay.sd);
   pb.parent = (int32_t*)(base + RLay.par);
@@ -246,11 +276,16 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       (void)hipGetLastError();
       return fail(SGC_ENOMEM, "pinned host flag staging of %llu bytes failed", (unsigned long long)sim_entries);
     }
-    SGC_HIP(hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming));
+    SGC_HIP(hipEventCreateWithFlags(&R.up_j[i], hipEventDisableTiming));
+    SGC_HIP(hipEventCreateWithFlags(&R.comp[i], hipEventDisableTiming));
+    SGC_HIP(hipEventCreateWithFlags(&R.back[i], hipEventDisableTiming));
     SGC_HIP(hipEventCreate(&R.t0[i]));
     SGC_HIP(hipEventCreate(&R.tm[i]));
     SGC_HIP(hipEventCreate(&R.t1[i]));
   }
+  SGC_HIP(hipEventCreateWithFlags(&R.up_i, hipEventDisableTiming));
+  SGC_HIP(hipStreamCreateWithFlags(&R.h2d, hipStreamNonBlocking));
+  SGC_HIP(hipStreamCreateWithFlags(&R.d2h, hipStreamNonBlocking));
   if (hipHostMalloc((void**)&R.flag, std::max<uint64_t>(g->m, 1)) != hipSuccess) {
     (void)hipGetLastError();
     return fail(SGC_ENOMEM, "pinned host flag array of %llu bytes failed", (unsigned long long)g->m);
@@ -262,105 +297,147 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   uint64_t totals[kCntSlots] = {0};
   double ms_sim = 0, ms_sim_large = 0;
 
-  // ---- similarity over chips of block pairs
+  // ---- similarity over chips of block pairs: uploads on R.h2d, kernels on
+  // the context stream, flag read-back on R.d2h, host merges in between
   const size_t P = blocks.size() - 1;
-  const SimChipArrays A = sim_arrays(R.chip, sim_entries, nullptr);
-  auto stage_block = [&](Staging& sg, size_t b, uint32_t base_off) -> int {
+  const SimChipArrays A = sim_arrays(R.chip, C, nullptr);
+  auto block_m = [&](size_t b) { return g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]]; };
+  auto stage_block = [&](Staging& sg, size_t b) -> int {
     if (sg.pending) SGC_HIP(hipEventSynchronize(sg.free));
     sg.pending = false;
     const double tb = now_ms();
     // a page-locked graph is sent as it lies; only the offsets are staged
-    ooc_copy_rows(g, blocks[b], blocks[b + 1], base_off, g->pinned ? nullptr : sg.adj, sg.off);
+    ooc_copy_rows(g, blocks[b], blocks[b + 1], 0, g->pinned ? nullptr : sg.adj, sg.off);
     R.ms_host += now_ms() - tb;
     return SGC_OK;
   };
-  auto upload_block = [&](Staging& sg, size_t b, int32_t* dst, uint32_t* off_dev) -> int {
-    const uint64_t m_b = g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]];
+  // block b's rows to chip entries [pos, pos + m_b), its offsets (from its
+  // first entry) to off_dev; `done` is recorded on the upload stream
+  auto upload_block = [&](Staging& sg, size_t b, uint64_t pos, uint32_t* off_dev, hipEvent_t done) -> int {
+    const uint64_t m_b = block_m(b);
     const uint32_t nr = (uint32_t)(blocks[b + 1] - blocks[b]);
     const int32_t* from = g->pinned ? g->adj + g->row_ptr[blocks[b]] : sg.adj;
-    if (m_b) SGC_HIP(hipMemcpyAsync(dst, from, m_b * 4, hipMemcpyHostToDevice, s));
-    SGC_HIP(hipMemcpyAsync(off_dev, sg.off, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
-    SGC_HIP(hipEventRecord(sg.free, s));
+    if (m_b) SGC_HIP(hipMemcpyAsync(A.adj + pos, from, m_b * 4, hipMemcpyHostToDevice, R.h2d));
+    SGC_HIP(hipMemcpyAsync(off_dev, sg.off, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, R.h2d));
+    SGC_HIP(hipEventRecord(sg.free, R.h2d));
     sg.pending = true;
+    SGC_HIP(hipEventRecord(done, R.h2d));
     R.bytes_h2d += (double)m_b * 4 + (double)(nr + 1) * 4;
-    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)blocks[b], nr, off_dev, A.src);
     return SGC_OK;
   };
+  bool used[2] = {false, false};  // comp[b] / back[b] recorded
   ChipDesc prev{};
+  int prev_b = 0;
   size_t chip_no = 0;
-  auto merge = [&](const ChipDesc& c, size_t k) -> int {
-    SGC_HIP(hipEventSynchronize(R.done[k & 1]));
+  auto merge = [&](const ChipDesc& c, int b) -> int {
+    SGC_HIP(hipEventSynchronize(R.back[b]));
     const double tb = now_ms();
-    ooc_merge_rows(g, c.r0i, c.r1i, R.h_flag[k & 1], R.flag);
-    if (c.r0j != c.r0i) ooc_merge_rows(g, c.r0j, c.r1j, R.h_flag[k & 1] + c.mi, R.flag);
-    for (int q = 0; q < kCntSlots; ++q) totals[q] += R.h_cnt[k & 1][q];
+    const uint8_t* hf = R.h_flag[b];
+    if (c.j_first) {
+      ooc_merge_rows(g, c.r0j, c.r1j, hf, R.flag);
+      ooc_merge_rows(g, c.r0i, c.r1i, hf + c.mj, R.flag);
+    } else {
+      ooc_merge_rows(g, c.r0i, c.r1i, hf, R.flag);
+      if (c.r0j != c.r0i) ooc_merge_rows(g, c.r0j, c.r1j, hf + c.mi, R.flag);
+    }
+    for (int q = 0; q < kCntSlots; ++q) totals[q] += R.h_cnt[b][q];
     R.ms_host += now_ms() - tb;
-    ms_sim += elapsed(R.t0[k & 1], R.t1[k & 1]);
-    ms_sim_large += elapsed(R.t0[k & 1], R.tm[k & 1]);
+    ms_sim += elapsed(R.t0[b], R.t1[b]);
+    ms_sim_large += elapsed(R.t0[b], R.tm[b]);
     return SGC_OK;
   };
   for (size_t i = 0; g->m && i < P; ++i) {
     ChipDesc c;
     c.r0i = blocks[i], c.r1i = blocks[i + 1];
-    c.mi = g->row_ptr[c.r1i] - g->row_ptr[c.r0i];
-    if ((rc = stage_block(R.stage[0], i, 0))) return rc;
-    if ((rc = upload_block(R.stage[0], i, A.adj, R.off_i))) return rc;
+    c.mi = block_m(i);
+    const uint32_t nri = (uint32_t)(c.r1i - c.r0i);
+    // block i goes to chip entries [C, C + mi) once every chip of the
+    // previous block i has run
+    if ((rc = stage_block(R.stage[0], i))) return rc;
+    for (int b = 0; b < 2; ++b)
+      if (used[b]) SGC_HIP(hipStreamWaitEvent(R.h2d, R.comp[b], 0));
+    if ((rc = upload_block(R.stage[0], i, C, R.off_i, R.up_i))) return rc;
+    SGC_HIP(hipStreamWaitEvent(s, R.up_i, 0));
+    if (nri) k_fill_src_rows<<<grid_for(nri, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0i, nri, R.off_i, A.src + C);
     for (size_t j = i; j < P; ++j) {
-      Staging& sj = R.stage[1 + (j & 1)];
+      const int b = (int)(chip_no & 1);
+      uint64_t base = C;  // the chip's first entry in adj / src
+      c.j_first = false;
+      uint32_t nrj = 0;
       if (j > i) {
         c.r0j = blocks[j], c.r1j = blocks[j + 1];
-        c.mj = g->row_ptr[c.r1j] - g->row_ptr[c.r0j];
-        // block j was staged (offsets based at mi) while chip (i, j-1) ran
-        if ((rc = upload_block(sj, j, A.adj + c.mi, R.off_j))) return rc;
+        c.mj = block_m(j);
+        nrj = (uint32_t)(c.r1j - c.r0j);
+        // block j right before block i (buffer 0) or right after it (buffer
+        // 1), once the chip before last, which used this buffer, has run
+        const uint64_t pos = b == 0 ? C - c.mj : C + c.mi;
+        Staging& sj = R.stage[1 + b];
+        if ((rc = stage_block(sj, j))) return rc;
+        if (used[b]) SGC_HIP(hipStreamWaitEvent(R.h2d, R.comp[b], 0));
+        if ((rc = upload_block(sj, j, pos, R.off_j[b], R.up_j[b]))) return rc;
+        SGC_HIP(hipStreamWaitEvent(s, R.up_j[b], 0));
+        if (nrj)
+          k_fill_src_rows<<<grid_for(nrj, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0j, nrj, R.off_j[b],
+                                                                           A.src + pos);
+        if (b == 0) {
+          base = pos;
+          c.j_first = true;
+        }
       } else {
         c.r0j = c.r0i, c.r1j = c.r1i, c.mj = 0;
       }
-      const uint32_t nri = (uint32_t)(c.r1i - c.r0i), nrj = (uint32_t)(c.r1j - c.r0j);
+      // flag array b is free once the chip before last has been read back
+      if (used[b]) SGC_HIP(hipStreamWaitEvent(s, R.back[b], 0));
       k_rows_absent<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, R.deg, R.rb, R.re);
-      if (nri) k_rows_range<<<grid_for(nri, kBlock), kBlock, 0, s>>>((uint32_t)c.r0i, nri, R.off_i, R.rb, R.re);
-      if (j > i && nrj)
-        k_rows_range<<<grid_for(nrj, kBlock), kBlock, 0, s>>>((uint32_t)c.r0j, nrj, R.off_j, R.rb, R.re);
+      if (nri)
+        k_rows_range<<<grid_for(nri, kBlock), kBlock, 0, s>>>((uint32_t)c.r0i, nri, c.j_first ? (uint32_t)c.mj : 0u,
+                                                              R.off_i, R.rb, R.re);
+      if (nrj)
+        k_rows_range<<<grid_for(nrj, kBlock), kBlock, 0, s>>>((uint32_t)c.r0j, nrj, c.j_first ? 0u : (uint32_t)c.mi,
+                                                              R.off_j[b], R.rb, R.re);
       const uint64_t m_chip = c.mi + c.mj;
       sgc_dgraph view;
       view.ctx = ctx;
       view.n = n;
       view.m = (uint32_t)m_chip;
-      view.adj = A.adj;
-      view.src = A.src;
+      view.adj = A.adj + base;
+      view.src = A.src + base;
       view.row_begin = R.rb;
       view.row_end = R.re;
       view.pair.lo0 = (uint32_t)c.r0i, view.pair.hi0 = (uint32_t)c.r1i;
       view.pair.lo1 = (uint32_t)c.r0j, view.pair.hi1 = (uint32_t)c.r1j;
       view.pair.on = true;
       PassBuffers cb = pb;
-      cb.flag = A.flag;
+      cb.flag = A.flag[b];
       cb.uedge = A.uedge;
       cb.units = A.units;
       cb.ucap = (uint32_t)unit_capacity(sim_entries);
-      const size_t k = chip_no;
       launch_init(&view, cb, s);
       launch_prep(&view, cb, p, s);
       launch_plan(&view, cb, p, s);
-      SGC_HIP(hipEventRecord(R.t0[k & 1], s));
-      launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[k & 1], s);
-      SGC_HIP(hipEventRecord(R.t1[k & 1], s));
+      SGC_HIP(hipEventRecord(R.t0[b], s));
+      launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[b], s);
+      SGC_HIP(hipEventRecord(R.t1[b], s));
       SGC_HIP(hipGetLastError());
-      if (m_chip) SGC_HIP(hipMemcpyAsync(R.h_flag[k & 1], A.flag, m_chip, hipMemcpyDeviceToHost, s));
-      SGC_HIP(hipMemcpyAsync(R.h_cnt[k & 1], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
-      SGC_HIP(hipEventRecord(R.done[k & 1], s));
+      // the counters before the next chip's init clears them
+      SGC_HIP(hipMemcpyAsync(R.h_cnt[b], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
+      SGC_HIP(hipEventRecord(R.comp[b], s));
+      SGC_HIP(hipStreamWaitEvent(R.d2h, R.comp[b], 0));
+      if (m_chip) SGC_HIP(hipMemcpyAsync(R.h_flag[b], A.flag[b], m_chip, hipMemcpyDeviceToHost, R.d2h));
+      SGC_HIP(hipEventRecord(R.back[b], R.d2h));
+      used[b] = true;
       R.bytes_d2h += (double)m_chip;
-      // while the device works: merge the previous chip, stage the next block
-      if (k > 0 && (rc = merge(prev, k - 1))) return rc;
-      if (j + 1 < P && j + 1 > i) {
-        Staging& nx = R.stage[1 + ((j + 1) & 1)];
-        if ((rc = stage_block(nx, j + 1, (uint32_t)c.mi))) return rc;
-      }
+      // while the device works: merge the previous chip
+      if (chip_no > 0 && (rc = merge(prev, prev_b))) return rc;
       prev = c;
+      prev_b = b;
       ++chip_no;
       if (chip_no % 32 == 0) progress("ooc: %zu of %zu similarity chips issued", chip_no, P * (P + 1) / 2);
     }
   }
-  if (chip_no && (rc = merge(prev, chip_no - 1))) return rc;
+  if (chip_no && (rc = merge(prev, prev_b))) return rc;
+  SGC_HIP(hipStreamSynchronize(R.h2d));
+  SGC_HIP(hipStreamSynchronize(s));
   if ((rc = check_unit_overflow(totals, sim_entries))) return rc;
   const size_t nsim = chip_no;
   progress("ooc: similarity done over %zu chips of %zu blocks", nsim, P);
@@ -496,7 +573,7 @@ extern "C" int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t*
   uint64_t maxd = 1;
   for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, g->row_ptr[x + 1] - g->row_ptr[x]);
   // a chip of two blocks that each hold the longest row
-  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + 2 * maxd * kSimChipBytesPerEntry + 4096;
+  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd) + 4096;
   return SGC_OK;
 }
 
