@@ -234,6 +234,9 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_server_graph_bytes", C.c_uint64, [_VP, C.c_uint32])
     _sig(lib, "sgc_server_bytes_in_use", C.c_uint64, [_VP])
     _sig(lib, "sgc_server_set_graph_launch", C.c_int, [_VP, C.c_int])
+    _sig(lib, "sgc_server_set_batching", C.c_int, [_VP, C.c_uint64, C.c_uint32])
+    _sig(lib, "sgc_server_get_batching", C.c_int, [_VP, _P(C.c_uint64), _P(C.c_uint32)])
+    _sig(lib, "sgc_server_autotune_batching", C.c_int, [_VP, C.c_double, C.c_int32, _P(C.c_uint64)])
     _sig(lib, "sgc_server_add_graph", C.c_int, [_VP, _VP, _P(C.c_uint32), _P(C.c_double)])
     _sig(lib, "sgc_server_remove_graph", C.c_int, [_VP, C.c_uint32])
     _sig(lib, "sgc_server_submit", C.c_int, [_VP, C.c_uint32, C.c_double, C.c_int32, _P(C.c_uint64)])
@@ -590,6 +593,23 @@ class Server:
     def set_graph_launch(self, enable: bool) -> None:
         """Replay a captured hipGraph per query (default) or launch eagerly."""
         _check(self._lib.sgc_server_set_graph_launch(self._h, 1 if enable else 0))
+
+    def set_batching(self, cutoff_entries: int, max_queries: int = 64) -> None:
+        """Queries on graphs of at most cutoff_entries entries run in batches
+        of up to max_queries, one launch per phase (0: off, the default)."""
+        _check(self._lib.sgc_server_set_batching(self._h, int(cutoff_entries), int(max_queries)))
+
+    def batching(self) -> tuple[int, int]:
+        cut, mx = C.c_uint64(), C.c_uint32()
+        _check(self._lib.sgc_server_get_batching(self._h, C.byref(cut), C.byref(mx)))
+        return int(cut.value), int(mx.value)
+
+    def autotune_batching(self, eps: float = 0.5, mu: int = 5) -> int:
+        """Times replay against batches per admitted graph and sets the
+        cut-off; returns it (entries)."""
+        cut = C.c_uint64()
+        _check(self._lib.sgc_server_autotune_batching(self._h, float(eps), int(mu), C.byref(cut)))
+        return int(cut.value)
 
     def add_graph(self, g: HostGraph) -> tuple[int, float]:
         gid = C.c_uint32()

@@ -7,6 +7,7 @@
 
 #include "device.h"
 #include "device_util.h"
+#include "scan_phases.h"
 
 namespace sgc {
 
@@ -278,18 +279,16 @@ __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t
                                                       int32_t* __restrict__ lo, int32_t* __restrict__ hi,
                                                       uint64_t* __restrict__ cnt) {
   if (mu_dev) mu = *mu_dev;
-  const uint32_t thr = mu >= 1 ? (uint32_t)mu : 1u;
+  PhaseView v{};
+  v.n = n;
+  v.sd = const_cast<uint32_t*>(sd);
+  v.rb = rb, v.re = re, v.core = core, v.parent = parent, v.label = label, v.lo = lo, v.hi = hi;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   uint64_t labeled = 0, marked = 0;
   for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
-    const bool c = sd[x] >= thr;
-    core[x] = c;
-    parent[x] = (int32_t)x;
-    label[x] = kNoLabel;
-    lo[x] = kNoLabel;  // classify's neighbour-label bounds
-    hi[x] = -1;
-    if (c && rb) labeled += re[x] - rb[x];
-    marked += c;
+    uint32_t mk;
+    labeled += core_mark_vertex(v, (uint32_t)x, mu, &mk);
+    marked += mk;
   }
   block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
   block_atomic_add((unsigned long long*)&cnt[kCntMarked], marked);
@@ -297,28 +296,8 @@ __global__ void __launch_bounds__(kBlock) k_core_mark(uint32_t n, const uint32_t
 
 // ---------------------------------------------------------------------------
 // Clusters: lock-free union-find, parents always smaller than children
-// (GPUSCAN++/GPUScan.cu:465-528, :632-652)
+// (GPUSCAN++/GPUScan.cu:465-528, :632-652; scan_phases.h)
 // ---------------------------------------------------------------------------
-
-__device__ int32_t uf_find(int32_t* parent, int32_t x) {
-  int32_t p = ld_coherent(parent + x);
-  while (p != x) {
-    const int32_t gp = ld_coherent(parent + p);
-    if (gp != p) st_coherent(parent + x, gp);  // path splitting
-    x = p;
-    p = gp;
-  }
-  return x;
-}
-
-__device__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
-  for (;;) {
-    const int32_t ra = uf_find(parent, a), rb = uf_find(parent, b);
-    if (ra == rb) return;
-    const int32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
-    if (atomicCAS(parent + hi, hi, lo) == hi) return;
-  }
-}
 
 // gate >= 0: runs only when cnt[gate] is not 0 (the union-find pass repeated
 // after the second similarity round of a pruned pass).
@@ -328,36 +307,19 @@ __global__ void __launch_bounds__(kBlock) k_union(uint32_t m, const uint32_t* __
                                                   const uint64_t* __restrict__ cnt, int gate) {
   if (cnt[kCntMarked] == 0) return;  // no core: no core-core edge to join
   if (gate >= 0 && cnt[gate] == 0) return;
-  constexpr uint32_t C = kEdgeChunks;
+  PhaseView v{};
+  v.m = m, v.src = src, v.adj = adj, v.flag = flag, v.core = const_cast<uint8_t*>(core), v.parent = parent;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
-    uint32_t u[C];
-    int32_t v[C];
-    bool y[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = k0 + j * stride;
-      y[j] = k < m && flag[k] == kFlagY;
-      u[j] = y[j] ? src[k] : 0u;
-      v[j] = y[j] ? adj[k] : 0;
-      y[j] = y[j] && (uint32_t)v[j] < u[j];
-    }
-    bool cc[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) cc[j] = y[j] && core[u[j]] && core[v[j]];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j)
-      if (cc[j]) uf_union(parent, (int32_t)u[j], v[j]);
-  }
+  for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * kEdgeChunks)
+    union_entries(v, k0, stride);
 }
 
-// Roots go to a separate array: the finds keep splitting paths in `parent`
-// (stores of ancestors only), which must not land on a finished label.
 __global__ void __launch_bounds__(kBlock) k_flatten(uint32_t n, const uint8_t* __restrict__ core, int32_t* parent,
                                                     int32_t* __restrict__ label) {
+  PhaseView v{};
+  v.n = n, v.core = const_cast<uint8_t*>(core), v.parent = parent, v.label = label;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride)
-    if (core[x]) label[x] = uf_find(parent, (int32_t)x);
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) flatten_vertex(v, (uint32_t)x);
 }
 
 // Deferred edges of a pruned pass that the later passes need, after the
@@ -415,47 +377,16 @@ __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* _
                                                    const uint32_t* __restrict__ re, int32_t* label,
                                                    uint64_t* __restrict__ cnt) {
   if (cnt[kCntMarked] == 0) return;  // no core: no border
-  constexpr uint32_t C = kEdgeChunks;
-  const uint32_t lane = lane_id();
+  PhaseView v{};
+  v.m = m, v.src = src, v.adj = adj, v.flag = flag, v.core = const_cast<uint8_t*>(core), v.rb = rb, v.re = re;
+  v.label = label;
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  // entries of the border rows this lane labels first (views without row
-  // offsets count nothing); the core rows' entries were counted by k_core_mark
+  // entries of the border rows this lane labels first; the core rows'
+  // entries were counted by k_core_mark
   uint32_t labeled = 0;
-  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
-       base += waves * kWave * C) {
-    // only similar entries can make a border: rows and neighbours are read
-    // for those alone.  The others get a marker row, so one row's lanes may
-    // form several runs; the segmented minimum may then also take lanes of
-    // the same row from another run, which are that row's candidates too,
-    // and each run's first lane applies it (an extra atomic at most).
-    bool y[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = base + j * kWave + lane;
-      y[j] = k < m && flag[k] == kFlagY;
-    }
-    uint32_t row[C];
-    int32_t c[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = base + j * kWave + lane;
-      row[j] = y[j] ? src[k] : 0xFFFFFFFFu;
-      c[j] = y[j] ? adj[k] : 0;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) y[j] = y[j] && core[c[j]] && !core[row[j]];
-    int32_t cand[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) cand[j] = y[j] ? label[c[j]] : kNoLabel;
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      if (!__ballot(y[j])) continue;
-      const uint32_t r = y[j] ? row[j] : 0xFFFFFFFFu;
-      const int32_t best = wave_seg_reduce(r, cand[j], OpMin());
-      if (wave_seg_head(r) && best != kNoLabel && atomicMin(&label[r], best) == kNoLabel && rb)
-        labeled += re[r] - rb[r];
-    }
-  }
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * kEdgeChunks);
+       base < m; base += waves * kWave * kEdgeChunks)
+    labeled += border_chunk(v, base);
   block_atomic_add((unsigned long long*)&cnt[kCntLabeled], labeled);
 }
 
@@ -463,70 +394,22 @@ __global__ void __launch_bounds__(kBlock) k_border(uint32_t m, const uint32_t* _
 // Hubs and outliers (GPUSCAN++/GPUScan.cu:732-798)
 // ---------------------------------------------------------------------------
 
-// For every vertex still unlabelled: min and max of its neighbours' labels.
-// Two different labels <=> the reference finds a first labelled neighbour and
-// a later one with another label.
+// Every launch over the graph's entries (m_all of them, maybe in several row
+// chips) takes the same side: from the labelled rows when they hold fewer
+// than a third of the entries.
 __global__ void __launch_bounds__(kBlock) k_classify(uint32_t m, uint64_t m_all, const uint32_t* __restrict__ src,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ label, int32_t* __restrict__ lo,
                                                      int32_t* __restrict__ hi, const uint64_t* __restrict__ cnt) {
   // no core means no label anywhere: lo and hi keep their initial values
   if (cnt[kCntMarked] == 0) return;
-  constexpr uint32_t C = kEdgeChunks;
-  const uint32_t lane = lane_id();
-  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  // Few entries in labelled rows: walk those rows and hand each label to the
-  // unlabelled neighbours (one gather per entry of a labelled row); else walk
-  // the unlabelled rows and gather their neighbours' labels.  Both give every
-  // unlabelled vertex the min and max label over its neighbours, as long as
-  // every launch over the graph's entries (m_all of them, maybe in several
-  // row chips) takes the same side.
+  PhaseView v{};
+  v.m = m, v.src = src, v.adj = adj, v.label = const_cast<int32_t*>(label), v.lo = lo, v.hi = hi;
   const bool from_labeled = 3 * cnt[kCntLabeled] < m_all;
-  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
-       base += waves * kWave * C) {
-    uint32_t row[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = base + j * kWave + lane;
-      row[j] = k < m ? src[k] : 0xFFFFFFFFu;
-    }
-    int32_t lr[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) lr[j] = row[j] != 0xFFFFFFFFu ? label[row[j]] : kNoLabel;
-    // neighbours only where this side needs them
-    int32_t nb[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j)
-      nb[j] = row[j] != 0xFFFFFFFFu && (from_labeled ? lr[j] != kNoLabel : lr[j] == kNoLabel)
-                  ? adj[base + j * kWave + lane]
-                  : 0;
-    if (from_labeled) {
-      bool to[C];
-#pragma unroll
-      for (uint32_t j = 0; j < C; ++j) to[j] = lr[j] != kNoLabel && label[nb[j]] == kNoLabel;
-#pragma unroll
-      for (uint32_t j = 0; j < C; ++j)
-        if (to[j]) {
-          atomicMin(&lo[nb[j]], lr[j]);
-          atomicMax(&hi[nb[j]], lr[j]);
-        }
-      continue;
-    }
-    int32_t ly[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) ly[j] = row[j] != 0xFFFFFFFFu && lr[j] == kNoLabel ? label[nb[j]] : kNoLabel;
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const bool has = ly[j] != kNoLabel;
-      if (!__ballot(has)) continue;
-      const int32_t vmin = wave_seg_reduce(row[j], has ? ly[j] : kNoLabel, OpMin());
-      const int32_t vmax = wave_seg_reduce(row[j], has ? ly[j] : -1, OpMax());
-      if (wave_seg_head(row[j]) && vmax >= 0) {
-        atomicMin(&lo[row[j]], vmin);
-        atomicMax(&hi[row[j]], vmax);
-      }
-    }
-  }
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * kEdgeChunks);
+       base < m; base += waves * kWave * kEdgeChunks)
+    classify_chunk(v, base, from_labeled);
 }
 
 // Also counts the similar entries: sd[x] holds those of row x.
@@ -535,36 +418,20 @@ __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t n, int32_t mu, con
                                                      const int32_t* __restrict__ hi, const uint8_t* __restrict__ core,
                                                      int32_t* __restrict__ label, uint64_t* __restrict__ cnt) {
   if (mu_dev) mu = *mu_dev;
-  uint32_t cores = 0, clusters = 0, borders = 0, hubs = 0, outliers = 0;
-  uint64_t similar = 0;
+  PhaseView v{};
+  v.n = n, v.sd = const_cast<uint32_t*>(sd), v.lo = const_cast<int32_t*>(lo), v.hi = const_cast<int32_t*>(hi);
+  v.core = const_cast<uint8_t*>(core), v.label = label;
+  RoleCounts rc;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
-    similar += sd[x];
-    int32_t l = label[x];
-    if (core[x]) {
-      ++cores;
-      clusters += l == (int32_t)x;
-    } else if (l != kNoLabel) {
-      ++borders;
-    } else {
-      // mu <= 0: the reference classifies only role -2 vertices and there
-      // are none, so every non-core keeps role -1 (GPUScan.cu:978-984).
-      const bool hub = mu <= 0 || (lo[x] != kNoLabel && lo[x] != hi[x]);
-      l = hub ? -1 : -2;
-      label[x] = l;
-      hubs += hub;
-      outliers += !hub;
-    }
-  }
-  block_atomic_add((unsigned long long*)&cnt[kCntCores], cores);
-  block_atomic_add((unsigned long long*)&cnt[kCntClusters], clusters);
-  block_atomic_add((unsigned long long*)&cnt[kCntBorders], borders);
-  block_atomic_add((unsigned long long*)&cnt[kCntHubs], hubs);
-  block_atomic_add((unsigned long long*)&cnt[kCntOutliers], outliers);
-  block_atomic_add((unsigned long long*)&cnt[kCntSimilar], similar);
+  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride)
+    finalize_vertex(v, (uint32_t)x, mu, &rc);
+  block_atomic_add((unsigned long long*)&cnt[kCntCores], rc.cores);
+  block_atomic_add((unsigned long long*)&cnt[kCntClusters], rc.clusters);
+  block_atomic_add((unsigned long long*)&cnt[kCntBorders], rc.borders);
+  block_atomic_add((unsigned long long*)&cnt[kCntHubs], rc.hubs);
+  block_atomic_add((unsigned long long*)&cnt[kCntOutliers], rc.outliers);
+  block_atomic_add((unsigned long long*)&cnt[kCntSimilar], rc.similar);
 }
-
-
 
 // Plain streaming copy (16 bytes per lane per step), the reference point for
 // the HBM byte counters.

@@ -28,6 +28,7 @@
 #include "device.h"
 #include "device_util.h"
 #include "graph_host.h"
+#include "scan_phases.h"
 
 namespace sgc {
 namespace {
@@ -45,8 +46,48 @@ struct QueryParams {
 // (GPUSCAN++/GPUScan.cu:143-169), then for undecided edges the predicate
 // `(cn+2)*(cn+2) < eps*eps*(s+1)*(l+1)` -> not similar (GPUScan.cu:220-221).
 // The same pass counts each row's similar entries into sd (the core count).
-// The neighbour degree comes from ndeg when the graph has it; eps comes from
-// prm when given.
+// The neighbour degree comes from ndeg when the graph has it.  One wave,
+// kEdgeChunks x 64 entries at `base`.
+__device__ __forceinline__ void threshold_chunk(uint32_t m, uint64_t base, const uint32_t* __restrict__ rp,
+                                                const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
+                                                const uint32_t* __restrict__ ndeg, const uint32_t* __restrict__ cn,
+                                                double eps, uint8_t* __restrict__ flag, uint32_t* __restrict__ sd) {
+  constexpr uint32_t C = kEdgeChunks;
+  const uint32_t lane = lane_id();
+  uint32_t u[C], v[C], du[C], dv[C];
+#pragma unroll
+  for (uint32_t j = 0; j < C; ++j) {
+    const uint64_t k = base + j * kWave + lane;
+    u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
+    v[j] = k < m ? (uint32_t)adj[k] : 0u;
+    dv[j] = k < m && ndeg ? ndeg[k] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < C; ++j) {
+    const bool in = u[j] != 0xFFFFFFFFu;
+    du[j] = in ? rp[u[j] + 1] - rp[u[j]] : 0u;
+    if (!ndeg) dv[j] = in ? rp[v[j] + 1] - rp[v[j]] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < C; ++j) {
+    const uint64_t k = base + j * kWave + lane;
+    uint32_t y = 0;
+    if (k < m) {
+      uint8_t f = prep_flag(u[j], v[j], du[j], dv[j], eps);
+      if (f >= kFlagUUp) {
+        const uint32_t s = f == kFlagUUp ? du[j] : dv[j], l = f == kFlagUUp ? dv[j] : du[j];
+        f = predicate_exact(cn[k], predicate_rhs(eps, s, l)) ? kFlagY : kFlagN;
+      }
+      flag[k] = f;
+      y = f == kFlagY;
+    }
+    if (!__ballot(y)) continue;
+    const uint32_t sum = wave_seg_reduce(u[j], y, OpAdd());
+    if (wave_seg_head(u[j]) && sum) atomicAdd(&sd[u[j]], sum);
+  }
+}
+
+// eps comes from prm when given (captured launches), else eps.
 __global__ void __launch_bounds__(kBlock) k_threshold(uint32_t m, const uint32_t* __restrict__ rp,
                                                       const uint32_t* __restrict__ src,
                                                       const int32_t* __restrict__ adj,
@@ -55,43 +96,10 @@ __global__ void __launch_bounds__(kBlock) k_threshold(uint32_t m, const uint32_t
                                                       const QueryParams* __restrict__ prm,
                                                       uint8_t* __restrict__ flag, uint32_t* __restrict__ sd) {
   if (prm) eps = prm->eps;
-  constexpr uint32_t C = kEdgeChunks;
-  const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * C); base < m;
-       base += waves * kWave * C) {
-    uint32_t u[C], v[C], du[C], dv[C];
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = base + j * kWave + lane;
-      u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
-      v[j] = k < m ? (uint32_t)adj[k] : 0u;
-      dv[j] = k < m && ndeg ? ndeg[k] : 0u;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const bool in = u[j] != 0xFFFFFFFFu;
-      du[j] = in ? rp[u[j] + 1] - rp[u[j]] : 0u;
-      if (!ndeg) dv[j] = in ? rp[v[j] + 1] - rp[v[j]] : 0u;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < C; ++j) {
-      const uint64_t k = base + j * kWave + lane;
-      uint32_t y = 0;
-      if (k < m) {
-        uint8_t f = prep_flag(u[j], v[j], du[j], dv[j], eps);
-        if (f >= kFlagUUp) {
-          const uint32_t s = f == kFlagUUp ? du[j] : dv[j], l = f == kFlagUUp ? dv[j] : du[j];
-          f = predicate_exact(cn[k], predicate_rhs(eps, s, l)) ? kFlagY : kFlagN;
-        }
-        flag[k] = f;
-        y = f == kFlagY;
-      }
-      if (!__ballot(y)) continue;
-      const uint32_t sum = wave_seg_reduce(u[j], y, OpAdd());
-      if (wave_seg_head(u[j]) && sum) atomicAdd(&sd[u[j]], sum);
-    }
-  }
+  for (uint64_t base = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * kEdgeChunks);
+       base < m; base += waves * kWave * kEdgeChunks)
+    threshold_chunk(m, base, rp, src, adj, ndeg, cn, eps, flag, sd);
 }
 
 __global__ void __launch_bounds__(kBlock) k_query_init(uint32_t n, uint32_t* __restrict__ sd, int32_t* __restrict__ lo,
@@ -104,6 +112,158 @@ __global__ void __launch_bounds__(kBlock) k_query_init(uint32_t n, uint32_t* __r
   }
   if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
 }
+
+// ---------------------------------------------------------------------------
+// Batched queries: one launch of each phase walks a table of queries
+// ---------------------------------------------------------------------------
+
+// Queries per batch, and the tiles the batched kernels cut each query into:
+// a block takes 1024 entries (four waves of kEdgeChunks x 64) or 256
+// vertices of one query, so block-wide reductions never mix two queries.
+constexpr uint32_t kBatchMax = 64;
+constexpr uint32_t kEdgeTile = (kBlock / kWave) * kWave * kEdgeChunks;
+constexpr uint32_t kVertexTile = kBlock;
+// Fixed grid of every batched kernel (grid-stride over the batch's tiles).
+constexpr uint32_t kBatchGrid = 1024;
+
+// One query of a batch: its graph, its slot's buffers, its parameters, the
+// host-visible outputs, and its first tile in the batch's edge and vertex
+// tile spaces.
+struct BatchItem {
+  const uint32_t* rp;
+  const uint32_t* src;
+  const int32_t* adj;
+  const uint32_t* ndeg;
+  const uint32_t* cn;
+  uint8_t* flag;
+  uint32_t* sd;
+  int32_t* parent;
+  int32_t* label;
+  int32_t* lo;
+  int32_t* hi;
+  uint8_t* core;
+  uint64_t* cnt;
+  int32_t* out_labels;  // the slot's page-locked label buffer, written by the device
+  uint64_t* out_cnt;    // the slot's page-locked counters
+  double eps;
+  int32_t mu;
+  uint32_t n, m;
+  uint32_t e_tile0, v_tile0;
+};
+
+struct BatchTable {
+  uint32_t count;    // queries in the batch
+  uint32_t e_tiles;  // edge tiles of all queries
+  uint32_t v_tiles;  // vertex tiles of all queries
+  uint32_t pad;
+  BatchItem items[kBatchMax];
+};
+
+// The query that tile t belongs to: the last one whose first tile is <= t
+// (first tiles increase with the query index), by one wave ballot.
+__device__ __forceinline__ uint32_t batch_item(const BatchTable* bt, uint32_t t, bool edge) {
+  const uint32_t lane = lane_id();
+  const bool in = lane < bt->count && (edge ? bt->items[lane].e_tile0 : bt->items[lane].v_tile0) <= t;
+  return (uint32_t)__popcll(__ballot(in)) - 1u;
+}
+
+__device__ __forceinline__ PhaseView item_view(const BatchItem& q) {
+  PhaseView v;
+  v.n = q.n, v.m = q.m, v.src = q.src, v.adj = q.adj, v.rb = q.rp, v.re = q.rp + 1, v.flag = q.flag;
+  v.sd = q.sd, v.core = q.core, v.parent = q.parent, v.label = q.label, v.lo = q.lo, v.hi = q.hi;
+  return v;
+}
+
+// Loops of the batched kernels: tile t of the batch, its query q and the
+// tile's first entry / vertex within q (block-uniform).
+#define SGC_BATCH_TILES(bt, tiles_field, edge, TILE)                                 \
+  for (uint32_t t = blockIdx.x; t < (bt)->tiles_field; t += gridDim.x)              \
+    if (const uint32_t q_ = batch_item((bt), t, (edge)); true)                      \
+      if (const BatchItem& Q = (bt)->items[q_]; true)                               \
+        if (const uint64_t T0 = (uint64_t)(t - ((edge) ? Q.e_tile0 : Q.v_tile0)) * (TILE); true)
+
+__global__ void __launch_bounds__(kBlock) k_batch_init(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, v_tiles, false, kVertexTile) {
+    const uint64_t x = T0 + threadIdx.x;
+    if (x < Q.n) {
+      Q.sd[x] = 0;
+      Q.lo[x] = kNoLabel;
+      Q.hi[x] = -1;
+    }
+    if (T0 == 0 && threadIdx.x < kCntSlots) Q.cnt[threadIdx.x] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_threshold(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, e_tiles, true, kEdgeTile) {
+    const uint64_t base = T0 + (threadIdx.x / kWave) * (kWave * kEdgeChunks);
+    threshold_chunk(Q.m, base, Q.rp, Q.src, Q.adj, Q.ndeg, Q.cn, Q.eps, Q.flag, Q.sd);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_core_mark(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, v_tiles, false, kVertexTile) {
+    const PhaseView v = item_view(Q);
+    const uint64_t x = T0 + threadIdx.x;
+    uint32_t marked = 0, labeled = 0;
+    if (x < Q.n) labeled = core_mark_vertex(v, (uint32_t)x, Q.mu, &marked);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntLabeled], labeled);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntMarked], marked);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_union(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, e_tiles, true, kEdgeTile) {
+    if (Q.cnt[kCntMarked] == 0) continue;  // no core: no core-core edge to join
+    union_entries(item_view(Q), T0 + threadIdx.x, kBlock);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_flatten(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, v_tiles, false, kVertexTile) {
+    const uint64_t x = T0 + threadIdx.x;
+    if (x < Q.n) flatten_vertex(item_view(Q), (uint32_t)x);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_border(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, e_tiles, true, kEdgeTile) {
+    if (Q.cnt[kCntMarked] == 0) continue;  // no core: no border
+    const uint32_t labeled = border_chunk(item_view(Q), T0 + (threadIdx.x / kWave) * (kWave * kEdgeChunks));
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntLabeled], labeled);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_classify(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, e_tiles, true, kEdgeTile) {
+    if (Q.cnt[kCntMarked] == 0) continue;  // no label anywhere
+    classify_chunk(item_view(Q), T0 + (threadIdx.x / kWave) * (kWave * kEdgeChunks), 3 * Q.cnt[kCntLabeled] < Q.m);
+  }
+}
+
+// Final labels straight into the slot's page-locked buffer, and the counts.
+__global__ void __launch_bounds__(kBlock) k_batch_finalize(const BatchTable* __restrict__ bt) {
+  SGC_BATCH_TILES(bt, v_tiles, false, kVertexTile) {
+    const PhaseView v = item_view(Q);
+    const uint64_t x = T0 + threadIdx.x;
+    RoleCounts rc;
+    if (x < Q.n) Q.out_labels[x] = finalize_vertex(v, (uint32_t)x, Q.mu, &rc);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntCores], rc.cores);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntClusters], rc.clusters);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntBorders], rc.borders);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntHubs], rc.hubs);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntOutliers], rc.outliers);
+    block_atomic_add((unsigned long long*)&Q.cnt[kCntSimilar], rc.similar);
+  }
+}
+
+// Each query's counters to its slot's page-locked copy (block q: query q).
+__global__ void __launch_bounds__(kBlock) k_batch_publish(const BatchTable* __restrict__ bt) {
+  for (uint32_t q = blockIdx.x; q < bt->count; q += gridDim.x)
+    if (threadIdx.x < kCntSlots) bt->items[q].out_cnt[threadIdx.x] = bt->items[q].cnt[threadIdx.x];
+}
+
+#undef SGC_BATCH_TILES
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -125,6 +285,11 @@ struct Slot {
   QueryParams* d_params = nullptr;  // device
   QueryParams* h_params = nullptr;  // pinned
   hipGraphExec_t exec = nullptr;
+  // batched: the slot's query runs in a batch lane (no per-slot events);
+  // the device addresses of h_labels / h_cnt, which batched kernels write
+  bool batched = false;
+  int32_t* d_out_labels = nullptr;
+  uint64_t* d_out_cnt = nullptr;
 };
 
 struct Graph {
@@ -150,6 +315,26 @@ struct Result {
   sgc_query_stats st;
 };
 
+// A lane runs one batch at a time: the table is written to page-locked
+// memory, and the lane's hipGraph (captured once: table upload, then every
+// batched kernel) is launched on the lane's stream.
+struct BatchMember {
+  Graph* G;
+  Slot* sl;
+  uint64_t ticket;
+  double t_submit;
+};
+struct BatchLane {
+  hipStream_t stream = nullptr;
+  BatchTable* h_table = nullptr;  // page-locked
+  BatchTable* d_table = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t start = nullptr, done = nullptr;
+  bool busy = false;
+  std::vector<BatchMember> members;
+};
+constexpr int kBatchLanes = 2;
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
@@ -173,6 +358,12 @@ struct sgc_server {
   std::deque<uint64_t> fresh;  // retired tickets not yet listed by sgc_server_completed
   std::unordered_map<uint64_t, uint32_t> inflight;  // ticket -> graph
   bool graph_launch = true;
+  // batching: queries on graphs of at most batch_cutoff entries run in
+  // batches of up to batch_max (0: off)
+  uint64_t batch_cutoff = 0;
+  uint32_t batch_max = sgc::kBatchMax;
+  bool lanes_ready = false;
+  sgc::BatchLane lanes[sgc::kBatchLanes];
 };
 
 namespace sgc {
@@ -235,6 +426,67 @@ int capture_slot(sgc_server* S, const Graph* G, Slot& sl) {
   return SGC_OK;
 }
 
+// Creates the batch lanes on first use: stream, table buffers, events, and
+// the lane's graph (table upload, then each batched kernel over the table's
+// tiles with a fixed grid), captured on the idle capture stream.
+int ensure_lanes(sgc_server* S) {
+  if (S->lanes_ready) return SGC_OK;
+  for (BatchLane& L : S->lanes) {
+    if (!L.stream) SGC_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    if (!L.h_table) SGC_HIP(hipHostMalloc((void**)&L.h_table, sizeof(BatchTable), hipHostMallocDefault));
+    if (!L.d_table) {
+      if (hipMalloc((void**)&L.d_table, sizeof(BatchTable)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SGC_ENOMEM, "batch table allocation failed");
+      }
+    }
+    if (!L.start) SGC_HIP(hipEventCreate(&L.start));
+    if (!L.done) SGC_HIP(hipEventCreate(&L.done));
+    memset(L.h_table, 0, sizeof(BatchTable));
+    if (!L.exec) {
+      hipStream_t st = S->capture_stream;
+      SGC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      (void)hipMemcpyAsync(L.d_table, L.h_table, sizeof(BatchTable), hipMemcpyHostToDevice, st);
+      k_batch_init<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_threshold<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_core_mark<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_union<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_flatten<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_border<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_classify<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_finalize<<<kBatchGrid, kBlock, 0, st>>>(L.d_table);
+      k_batch_publish<<<kBatchMax, kBlock, 0, st>>>(L.d_table);
+      const hipError_t le = hipGetLastError();
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(SGC_EDEVICE, "capture of the batch graph failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return fail(SGC_EDEVICE, "batch graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+  }
+  S->lanes_ready = true;
+  return SGC_OK;
+}
+
+void free_lanes(sgc_server* S) {
+  for (BatchLane& L : S->lanes) {
+    if (L.stream) (void)hipStreamSynchronize(L.stream);
+    if (L.exec) (void)hipGraphExecDestroy(L.exec);
+    if (L.h_table) (void)hipHostFree(L.h_table);
+    if (L.d_table) (void)hipFree(L.d_table);
+    if (L.start) (void)hipEventDestroy(L.start);
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    L = BatchLane();
+  }
+  S->lanes_ready = false;
+}
+
 // Launches query q into free slot s of graph G.
 int launch_query(sgc_server* S, Graph* G, Slot& sl, const Query& q) {
   hipStream_t st = S->streams[sl.stream];
@@ -257,9 +509,73 @@ int launch_query(sgc_server* S, Graph* G, Slot& sl, const Query& q) {
   return SGC_OK;
 }
 
-// Starts queued queries in free slots (FIFO per graph, skipping queries whose
-// graph is full so other graphs keep the device busy).
+bool batchable(const sgc_server* S, const Graph* G) { return S->batch_cutoff && G->dg->m <= S->batch_cutoff; }
+
+// Fills lane's table with its members' queries and launches the lane.
+int launch_batch(sgc_server* S, BatchLane& L, const std::vector<double>& eps, const std::vector<int32_t>& mu) {
+  BatchTable* T = L.h_table;
+  uint32_t et = 0, vt = 0;
+  T->count = (uint32_t)L.members.size();
+  for (uint32_t q = 0; q < T->count; ++q) {
+    const BatchMember& b = L.members[q];
+    const sgc_dgraph* g = b.G->dg;
+    const PassBuffers& pb = b.sl->pb;
+    BatchItem& it = T->items[q];
+    it.rp = g->row_ptr, it.src = g->src, it.adj = g->adj, it.ndeg = g->ndeg, it.cn = b.G->cn;
+    it.flag = pb.flag, it.sd = pb.sd, it.parent = pb.parent, it.label = pb.label, it.lo = pb.lo, it.hi = pb.hi;
+    it.core = pb.core, it.cnt = pb.cnt;
+    it.out_labels = b.sl->d_out_labels, it.out_cnt = b.sl->d_out_cnt;
+    it.eps = eps[q], it.mu = mu[q], it.n = g->n, it.m = g->m;
+    it.e_tile0 = et, it.v_tile0 = vt;
+    et += (uint32_t)(((uint64_t)g->m + kEdgeTile - 1) / kEdgeTile);
+    vt += (uint32_t)std::max<uint64_t>(1, ((uint64_t)g->n + kVertexTile - 1) / kVertexTile);  // its counters are cleared there
+  }
+  T->e_tiles = et;
+  T->v_tiles = vt;
+  SGC_HIP(hipEventRecord(L.start, L.stream));
+  SGC_HIP(hipGraphLaunch(L.exec, L.stream));
+  SGC_HIP(hipEventRecord(L.done, L.stream));
+  L.busy = true;
+  for (const BatchMember& b : L.members) S->inflight[b.ticket] = b.G->id;
+  return SGC_OK;
+}
+
+// Starts queued queries: queries on batchable graphs go to free batch lanes
+// (each batch takes the oldest queued ones whose graph has a free slot, up to
+// batch_max), the others to free slots of their graph (FIFO per graph,
+// skipping queries whose graph is full so other graphs keep the device busy).
 int dispatch(sgc_server* S) {
+  for (BatchLane& L : S->lanes) {
+    if (!S->batch_cutoff || !S->lanes_ready || L.busy) continue;
+    L.members.clear();
+    std::vector<double> eps;
+    std::vector<int32_t> mu;
+    for (auto it = S->pending.begin(); it != S->pending.end() && L.members.size() < S->batch_max;) {
+      auto gi = S->graphs.find(it->graph);
+      Graph* G = gi == S->graphs.end() ? nullptr : gi->second;
+      Slot* free_slot = nullptr;
+      if (G && batchable(S, G))
+        for (Slot& sl : G->slots)
+          if (!sl.busy) {
+            free_slot = &sl;
+            break;
+          }
+      if (!free_slot) {
+        ++it;
+        continue;
+      }
+      free_slot->busy = true;
+      free_slot->batched = true;
+      free_slot->ticket = it->ticket;
+      free_slot->t_submit = it->t_submit;
+      L.members.push_back(BatchMember{G, free_slot, it->ticket, it->t_submit});
+      eps.push_back(it->eps);
+      mu.push_back(it->mu);
+      it = S->pending.erase(it);
+    }
+    if (L.members.empty()) break;
+    if (int rc = launch_batch(S, L, eps, mu)) return rc;
+  }
   for (auto it = S->pending.begin(); it != S->pending.end();) {
     auto gi = S->graphs.find(it->graph);
     if (gi == S->graphs.end()) {  // graph removed: drop with an error result
@@ -271,6 +587,10 @@ int dispatch(sgc_server* S) {
       continue;
     }
     Graph* G = gi->second;
+    if (batchable(S, G) && S->lanes_ready) {  // waits for a batch lane
+      ++it;
+      continue;
+    }
     Slot* free_slot = nullptr;
     for (Slot& s : G->slots)
       if (!s.busy) {
@@ -290,39 +610,65 @@ int dispatch(sgc_server* S) {
 
 // Retires finished slots; with wait, blocks for the oldest in-flight query
 // first.  Returns the number retired.
+Result slot_result(const Graph* G, const Slot& s, double ms_device) {
+  Result r;
+  r.labels.assign(s.h_labels, s.h_labels + G->dg->n);
+  memset(&r.st, 0, sizeof(r.st));
+  r.st.status = SGC_OK;
+  r.st.graph = G->id;
+  r.st.n = G->dg->n;
+  r.st.ms_device = ms_device;
+  r.st.ms_latency = now_ms() - s.t_submit;
+  const uint64_t* c = s.h_cnt;
+  r.st.similar_edges = c[kCntSimilar] / 2;
+  r.st.cores = c[kCntCores];
+  r.st.clusters = c[kCntClusters];
+  r.st.borders = c[kCntBorders];
+  r.st.hubs = c[kCntHubs];
+  r.st.outliers = c[kCntOutliers];
+  return r;
+}
+
 int retire(sgc_server* S, bool wait, int* retired) {
   *retired = 0;
   if (wait) {
-    // the in-flight query submitted first
-    Slot* oldest = nullptr;
+    // the in-flight query submitted first (its slot's, or its batch's, event)
+    hipEvent_t oldest = nullptr;
+    uint64_t t_old = ~0ull;
     for (auto& kv : S->graphs)
       for (Slot& s : kv.second->slots)
-        if (s.busy && (!oldest || s.ticket < oldest->ticket)) oldest = &s;
-    if (oldest) SGC_HIP(hipEventSynchronize(oldest->done));
+        if (s.busy && !s.batched && s.ticket < t_old) t_old = s.ticket, oldest = s.done;
+    for (BatchLane& L : S->lanes)
+      if (L.busy)
+        for (const BatchMember& b : L.members)
+          if (b.ticket < t_old) t_old = b.ticket, oldest = L.done;
+    if (oldest) SGC_HIP(hipEventSynchronize(oldest));
+  }
+  for (BatchLane& L : S->lanes) {
+    if (!L.busy) continue;
+    const hipError_t e = hipEventQuery(L.done);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return fail(SGC_EDEVICE, "batch failed: %s", hipGetErrorString(e));
+    const double ms = elapsed(L.start, L.done);
+    for (const BatchMember& b : L.members) {
+      S->done.emplace(b.ticket, slot_result(b.G, *b.sl, ms));
+      S->fresh.push_back(b.ticket);
+      S->inflight.erase(b.ticket);
+      b.sl->busy = false;
+      b.sl->batched = false;
+      ++*retired;
+    }
+    L.members.clear();
+    L.busy = false;
   }
   for (auto& kv : S->graphs) {
     Graph* G = kv.second;
     for (Slot& s : G->slots) {
-      if (!s.busy) continue;
+      if (!s.busy || s.batched) continue;
       const hipError_t e = hipEventQuery(s.done);
       if (e == hipErrorNotReady) continue;
       if (e != hipSuccess) return fail(SGC_EDEVICE, "query failed: %s", hipGetErrorString(e));
-      Result r;
-      r.labels.assign(s.h_labels, s.h_labels + G->dg->n);
-      memset(&r.st, 0, sizeof(r.st));
-      r.st.status = SGC_OK;
-      r.st.graph = G->id;
-      r.st.n = G->dg->n;
-      r.st.ms_device = elapsed(s.start, s.done);
-      r.st.ms_latency = now_ms() - s.t_submit;
-      const uint64_t* c = s.h_cnt;
-      r.st.similar_edges = c[kCntSimilar] / 2;
-      r.st.cores = c[kCntCores];
-      r.st.clusters = c[kCntClusters];
-      r.st.borders = c[kCntBorders];
-      r.st.hubs = c[kCntHubs];
-      r.st.outliers = c[kCntOutliers];
-      S->done.emplace(s.ticket, std::move(r));
+      S->done.emplace(s.ticket, slot_result(G, s, elapsed(s.start, s.done)));
       S->fresh.push_back(s.ticket);
       S->inflight.erase(s.ticket);
       s.busy = false;
@@ -361,6 +707,7 @@ extern "C" void sgc_server_destroy(sgc_server* S) {
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
   for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(S->streams[i]);
+  free_lanes(S);
   for (auto& kv : S->graphs) free_graph(kv.second);
   for (int i = 0; i < kStreams; ++i) (void)hipStreamDestroy(S->streams[i]);
   if (S->capture_stream) (void)hipStreamDestroy(S->capture_stream);
@@ -470,6 +817,14 @@ extern "C" int sgc_server_add_graph(sgc_server* S, const sgc_host_graph* hg, uin
       free_graph(G);
       return fail(SGC_ENOMEM, "query slot set-up failed");
     }
+    // the device's view of the slot's page-locked outputs (batched kernels
+    // write the labels and counters there directly)
+    if (hipHostGetDevicePointer((void**)&sl.d_out_labels, sl.h_labels, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&sl.d_out_cnt, sl.h_cnt, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      free_graph(G);
+      return fail(SGC_EDEVICE, "device address of a query slot's host buffers failed");
+    }
     // the slot's query work as one replayable graph (used when graph launch
     // is on), captured now on the idle capture stream
     if ((rc = capture_slot(S, G, sl))) {
@@ -493,7 +848,9 @@ extern "C" int sgc_server_remove_graph(sgc_server* S, uint32_t graph_id) {
   if (it == S->graphs.end()) return fail(SGC_EINVAL, "unknown graph %u", graph_id);
   Graph* G = it->second;
   for (Slot& s : G->slots)
-    if (s.busy) SGC_HIP(hipEventSynchronize(s.done));
+    if (s.busy && !s.batched) SGC_HIP(hipEventSynchronize(s.done));
+  for (BatchLane& L : S->lanes)
+    if (L.busy) SGC_HIP(hipEventSynchronize(L.done));
   int retired = 0;
   int rc = retire(S, false, &retired);  // keep the finished results
   if (rc) return rc;
@@ -509,6 +866,76 @@ extern "C" int sgc_server_set_graph_launch(sgc_server* S, int enable) {
   clear_error();
   if (!S) return fail(SGC_EINVAL, "null server");
   S->graph_launch = enable != 0;
+  return SGC_OK;
+}
+
+extern "C" int sgc_server_set_batching(sgc_server* S, uint64_t cutoff_entries, uint32_t max_queries) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  if (max_queries < 1 || max_queries > kBatchMax)
+    return fail(SGC_EINVAL, "max_queries %u outside [1, %u]", max_queries, kBatchMax);
+  SGC_HIP(hipSetDevice(S->ctx->device));
+  if (cutoff_entries) {
+    if (int rc = ensure_lanes(S)) return rc;
+  } else {
+    // queries already in lanes finish there; queued ones go to slots
+  }
+  S->batch_cutoff = cutoff_entries;
+  S->batch_max = max_queries;
+  return SGC_OK;
+}
+
+extern "C" int sgc_server_get_batching(const sgc_server* S, uint64_t* cutoff_entries, uint32_t* max_queries) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  if (cutoff_entries) *cutoff_entries = S->batch_cutoff;
+  if (max_queries) *max_queries = S->batch_max;
+  return SGC_OK;
+}
+
+extern "C" int sgc_server_autotune_batching(sgc_server* S, double eps, int32_t mu, uint64_t* cutoff_entries) {
+  clear_error();
+  if (!S) return fail(SGC_EINVAL, "null server");
+  if (!S->pending.empty() || !S->inflight.empty()) return fail(SGC_EBUSY, "queries are still open");
+  SGC_HIP(hipSetDevice(S->ctx->device));
+  if (int rc = ensure_lanes(S)) return rc;
+  const bool launch_mode = S->graph_launch;
+  S->graph_launch = true;
+  BatchLane& L = S->lanes[0];
+  uint64_t best = 0;
+  int rc = SGC_OK;
+  for (auto& kv : S->graphs) {
+    Graph* G = kv.second;
+    // one query replayed in its slot (second of two runs)
+    Slot& sl = G->slots[0];
+    const Query q{0, G->id, eps, mu, now_ms()};
+    double t_replay = 0, t_batch = 0;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+      rc = launch_query(S, G, sl, q);
+      if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = fail(SGC_EDEVICE, "autotune replay failed");
+      sl.busy = false;
+      S->inflight.erase(0);
+      t_replay = elapsed(sl.start, sl.done);
+    }
+    // the graph's slots as one batch, per query (second of two runs)
+    const uint32_t k = std::min<uint32_t>((uint32_t)G->slots.size(), S->batch_max);
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+      L.members.clear();
+      for (uint32_t i = 0; i < k; ++i) L.members.push_back(BatchMember{G, &G->slots[i], 0, now_ms()});
+      rc = launch_batch(S, L, std::vector<double>(k, eps), std::vector<int32_t>(k, mu));
+      if (!rc && hipEventSynchronize(L.done) != hipSuccess) rc = fail(SGC_EDEVICE, "autotune batch failed");
+      S->inflight.erase(0);
+      L.busy = false;
+      t_batch = elapsed(L.start, L.done) / k;
+    }
+    L.members.clear();
+    if (rc) break;
+    if (t_batch < t_replay) best = std::max<uint64_t>(best, G->dg->m);
+  }
+  S->graph_launch = launch_mode;
+  if (rc) return rc;
+  S->batch_cutoff = best;
+  if (cutoff_entries) *cutoff_entries = best;
   return SGC_OK;
 }
 

@@ -343,6 +343,21 @@ uint64_t sgc_server_bytes_in_use(const sgc_server* srv);
  * once into a hipGraph and replayed per query with eps/mu read from device
  * memory; 0 = eager kernel launches. */
 int sgc_server_set_graph_launch(sgc_server* srv, int enable);
+/* Batching: queries on graphs of at most `cutoff_entries` directed entries
+ * are not replayed one per slot but gathered, oldest first, into batches of
+ * up to `max_queries` (1..64), each run by one launch of every phase kernel
+ * that walks a table of (graph, eps, mu) queries (a hipGraph captured once
+ * per batch lane; two lanes run at once).  Results are the same as a slot's.
+ * cutoff 0 (the default) turns batching off.  No counterpart in the
+ * reference. */
+int sgc_server_set_batching(sgc_server* srv, uint64_t cutoff_entries, uint32_t max_queries);
+int sgc_server_get_batching(const sgc_server* srv, uint64_t* cutoff_entries, uint32_t* max_queries);
+/* Picks the batching cut-off from measurement: for every admitted graph, the
+ * device time of one (eps, mu) query replayed in a slot against the device
+ * time per query of a batch of its slots; the cut-off becomes the largest
+ * graph (in entries) for which the batch is faster per query, 0 if none.
+ * Needs an idle server (SGC_EBUSY otherwise). */
+int sgc_server_autotune_batching(sgc_server* srv, double eps, int32_t mu, uint64_t* cutoff_entries);
 /* Admission: uploads g and computes every edge's common-neighbour count once
  * (precompute_ms: that pass).  SGC_ENOMEM if the budget cannot hold it; the
  * caller may remove graphs and retry. */
