@@ -152,6 +152,13 @@ struct PassBuffers {
   // state (kVUndecided / kVCore / kVNonCore) in core's.
   uint32_t* ny;
   uint32_t* nn;
+  // Packed edge records of out-of-core chips (launch_prep_packed): each
+  // owner's first record (n, the exclusive prefix of its owned count) and
+  // its cursor (n), and the scan's scratch.
+  uint32_t* rec_start = nullptr;
+  uint32_t* rec_cur = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
   uint32_t ucap;
   uint64_t* cnt;     // kCntSlots
 };
@@ -198,6 +205,18 @@ void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);
 // With p.prune, also counts each row's degree-decided similar and dissimilar
 // entries (ny, nn).
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all = false);
+// Scratch bytes of the exclusive scan over n per-owner counts (packed records).
+size_t scan_scratch_bytes(uint32_t n);
+// Packed edge records, for chips whose record array holds fewer records than
+// the chip has entries: the prep kernel writes flags and per-owner counts,
+// the counts are scanned into b.rec_start (and copied to b.rec_cur); then
+// launch_similarity_packed decides the owned edges in `rounds` rounds, round
+// r taking the owners whose records start in [r w, (r + 1) w), packed into
+// b.uedge (which must hold w plus the largest owned count); rounds past the
+// chip's records return at once.
+void launch_prep_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
+void launch_similarity_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
+                              hipEvent_t mid, hipStream_t s, uint32_t w, uint32_t rounds);
 // With p.prune, also sets every vertex's state from those counts.
 void launch_plan(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s);
 // Second similarity round of a pruned pass (after the union-find pass): the

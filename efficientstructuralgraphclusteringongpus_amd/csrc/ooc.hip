@@ -82,20 +82,22 @@ struct Layout {
 };
 
 // Similarity chips are two row blocks of at most C entries.  Their device
-// memory, per block entry: adj and src as [ j-buffer 0 | block i | j-buffer 1 ]
-// (3 x 4 B each; the second block of a chip sits right before the first in
-// buffer 0 or right after it in buffer 1, so every chip is one contiguous
-// entry range and the next chip's second block uploads while this one runs),
-// two flag arrays of 2C (consecutive chips alternate, so one chip's flags go
-// back to the host while the next one computes), the edge records (2C x 16 B)
-// and work units (C x 16 B) of one chip.  Row chips of the later phases hold
-// adj, src and flag.
-constexpr uint64_t kSimBytesPerBlockEntry = 3 * 4 + 3 * 4 + 2 * 2 + 2 * 16 + 16;
+// memory: adj and src as [ j-buffer 0 | block i | j-buffer 1 ] (3C entries
+// each; the second block of a chip sits right before the first in buffer 0
+// or right after it in buffer 1, so every chip is one contiguous entry range
+// and the next chip's second block uploads while this one runs), two flag
+// arrays of 2C (consecutive chips alternate, so one chip's flags go back to
+// the host while the next one computes), and R packed edge records and R + 2
+// work units (16 B each) for one round of owners.  A chip owns at most C
+// undecided edges (each has both entries in the chip), decided in rounds of
+// owners whose records start in a window of W = R - (largest degree), so
+// R = C / 2 needs two or three rounds: 28 + 32 R / C = 44 B per block entry.
+constexpr uint64_t kSimBytesPerBlockEntry = 3 * 4 + 3 * 4 + 2 * 2 + 16;  // with R = C / 2
 constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
 struct ResidentLayout {
-  size_t deg, rb, re, off_i, off_j0, off_j1, sd, par, lab, lo, hi, core, cntu, work, cnt, bytes;
+  size_t deg, rb, re, off_i, off_j0, off_j1, sd, par, lab, lo, hi, core, cntu, work, cnt, scan, scan_bytes, bytes;
 };
 
 ResidentLayout resident_layout(uint32_t n) {
@@ -107,6 +109,8 @@ ResidentLayout resident_layout(uint32_t n) {
   r.sd = L.take((size_t)n * 4), r.par = L.take((size_t)n * 4), r.lab = L.take((size_t)n * 4);
   r.lo = L.take((size_t)n * 4), r.hi = L.take((size_t)n * 4), r.core = L.take(n);
   r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.cnt = L.take(kCntSlots * 8);
+  r.scan_bytes = std::max<size_t>(scan_scratch_bytes(n), 1);
+  r.scan = L.take(r.scan_bytes);
   r.bytes = L.bytes;
   return r;
 }
@@ -115,14 +119,14 @@ struct SimChipArrays {
   int32_t* adj;      // 3C
   uint32_t* src;     // 3C
   uint8_t* flag[2];  // 2C each
-  uint4* uedge;      // 2C
-  uint4* units;      // unit_capacity(2C)
+  uint4* uedge;      // R
+  uint4* units;      // R + 2
 };
 
-SimChipArrays sim_arrays(char* base, uint64_t C, size_t* bytes) {
+SimChipArrays sim_arrays(char* base, uint64_t C, uint64_t R, size_t* bytes) {
   Layout L;
   const size_t o_adj = L.take(3 * C * 4), o_src = L.take(3 * C * 4), o_f0 = L.take(2 * C), o_f1 = L.take(2 * C),
-               o_uedge = L.take(2 * C * sizeof(uint4)), o_units = L.take(unit_capacity(2 * C) * sizeof(uint4));
+               o_uedge = L.take(R * sizeof(uint4)), o_units = L.take((R + 2) * sizeof(uint4));
   if (bytes) *bytes = L.bytes;
   return SimChipArrays{(int32_t*)(base + o_adj),
                        (uint32_t*)(base + o_src),
@@ -131,10 +135,18 @@ SimChipArrays sim_arrays(char* base, uint64_t C, size_t* bytes) {
                        (uint4*)(base + o_units)};
 }
 
-size_t sim_chip_bytes(uint64_t C) {
+size_t sim_chip_bytes(uint64_t C, uint64_t R) {
   size_t b = 0;
-  sim_arrays(nullptr, C, &b);
+  sim_arrays(nullptr, C, R, &b);
   return b;
+}
+
+// Records per chip for blocks of C entries and rows of at most maxd entries:
+// a window of at least C / 2 - maxd (and at least C / 4) plus the largest
+// owner's count.
+uint64_t record_capacity(uint64_t C, uint64_t maxd) {
+  const uint64_t half = C / 2, w = half > maxd + C / 4 ? half - maxd : std::max<uint64_t>(C / 4, 1);
+  return w + maxd;
 }
 
 size_t row_chip_bytes(uint64_t cap) {
@@ -220,20 +232,33 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     return fail(SGC_ENOMEM, "device budget %llu bytes is below the %llu resident bytes the graph's %u vertices need",
                 (unsigned long long)budget, (unsigned long long)resident, n);
   const uint64_t room = budget - resident - 8192;
-  const uint64_t cap_block = std::min<uint64_t>(room / kSimBytesPerBlockEntry, kMaxChipEntries / 2);
+  uint64_t cap_block = std::min<uint64_t>(room / kSimBytesPerBlockEntry, kMaxChipEntries / 2);
   const uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries);
   double t0 = now_ms();
   std::vector<uint32_t> deg(n);
   ooc_degrees(g, deg.data());
   std::vector<uint64_t> blocks, row_bounds;
   uint64_t block_max = 0, row_max = 0;
-  int rc = ooc_plan_rows(g, cap_block, &blocks, &block_max);
-  if (rc) return rc;
+  uint64_t maxd = 1;
+  for (uint32_t x = 0; x < n; ++x) maxd = std::max<uint64_t>(maxd, deg[x]);
+  // blocks as large as the room allows once the records of the largest
+  // degree are counted in
+  int rc = SGC_OK;
+  uint64_t C = 1, RC = 1;  // entries per block, records per chip at most
+  for (;;) {
+    rc = ooc_plan_rows(g, cap_block, &blocks, &block_max);
+    if (rc) return rc;
+    C = std::max<uint64_t>(block_max, 1);
+    RC = record_capacity(C, maxd);
+    if (sim_chip_bytes(C, RC) <= room || cap_block <= maxd) break;
+    cap_block = std::max<uint64_t>(maxd, cap_block - cap_block / 8);
+  }
   rc = ooc_plan_rows(g, cap_row, &row_bounds, &row_max);
   if (rc) return rc;
-  const uint64_t C = std::max<uint64_t>(block_max, 1);  // entries per block at most
-  const uint64_t sim_entries = 2 * C;                   // entries per chip at most
-  const size_t chip_need = std::max(sim_chip_bytes(C), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
+  const uint64_t W = RC - maxd;                // window of one round of owners
+  const uint32_t rounds = (uint32_t)((C + W - 1) / W);  // a chip owns at most C edges
+  const uint64_t sim_entries = 2 * C;          // entries per chip at most
+  const size_t chip_need = std::max(sim_chip_bytes(C, RC), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
   R.ms_host += now_ms() - t0;
 
   // ---- device and pinned host memory
@@ -259,6 +284,12 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   pb.cnt_u = (uint32_t*)(base + RLay.cntu);
   pb.work_u = (uint64_t*)(base + RLay.work);
   pb.cnt = (uint64_t*)(base + RLay.cnt);
+  // packed records of the similarity chips: first record and cursor of each
+  // owner in lo's and sd's memory (free until the core phase)
+  pb.rec_start = (uint32_t*)pb.lo;
+  pb.rec_cur = pb.sd;
+  pb.scan_tmp = base + RLay.scan;
+  pb.scan_tmp_bytes = RLay.scan_bytes;
   R.chip = base + resident;
 
   const uint64_t stage_entries = std::max<uint64_t>(block_max, 1);
@@ -300,7 +331,7 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   // ---- similarity over chips of block pairs: uploads on R.h2d, kernels on
   // the context stream, flag read-back on R.d2h, host merges in between
   const size_t P = blocks.size() - 1;
-  const SimChipArrays A = sim_arrays(R.chip, C, nullptr);
+  const SimChipArrays A = sim_arrays(R.chip, C, RC, nullptr);
   auto block_m = [&](size_t b) { return g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]]; };
   auto stage_block = [&](Staging& sg, size_t b) -> int {
     if (sg.pending) SGC_HIP(hipEventSynchronize(sg.free));
@@ -411,12 +442,11 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       cb.flag = A.flag[b];
       cb.uedge = A.uedge;
       cb.units = A.units;
-      cb.ucap = (uint32_t)unit_capacity(sim_entries);
+      cb.ucap = (uint32_t)(RC + 2);
       launch_init(&view, cb, s);
-      launch_prep(&view, cb, p, s);
-      launch_plan(&view, cb, p, s);
+      launch_prep_packed(&view, cb, p, s);
       SGC_HIP(hipEventRecord(R.t0[b], s));
-      launch_similarity(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[b], s);
+      launch_similarity_packed(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[b], s, (uint32_t)W, rounds);
       SGC_HIP(hipEventRecord(R.t1[b], s));
       SGC_HIP(hipGetLastError());
       // the counters before the next chip's init clears them
@@ -573,7 +603,7 @@ extern "C" int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t*
   uint64_t maxd = 1;
   for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, g->row_ptr[x + 1] - g->row_ptr[x]);
   // a chip of two blocks that each hold the longest row
-  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd) + 4096;
+  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd, record_capacity(maxd, maxd)) + 4096;
   return SGC_OK;
 }
 

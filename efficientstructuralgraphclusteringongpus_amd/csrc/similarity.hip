@@ -23,6 +23,8 @@
 //    window at a time as an exact 128 KB bitmap.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_scan.hpp>
+
 #include "device.h"
 #include "device_util.h"
 
@@ -78,7 +80,19 @@ __device__ __forceinline__ uint32_t log2_pow2(uint32_t t) { return 31 - __clz(t)
 // pruned pass): the degree-only test is not rerun; the entries the round
 // marked kFlagUDown are planned as owned, their count goes to
 // cnt[kCntRedoOwned], and the kernel returns at once when cnt[kCntRedo] is 0.
-template <bool COUNTS, bool REDO>
+// Where k_prep puts each owner's edge records:
+//   kRecRow     at the owner's own row range, uedge[rb[b] + slot];
+//   kRecCount   nowhere: flags and per-owner counts only (first of two passes);
+//   kRecCompact packed by owner: owner b's records start at rec_start[b],
+//               its exclusive prefix of the counts (second pass: the flags of
+//               the first are read back, owned = kFlagUDown).  One round
+//               writes the owners whose rec_start lies in [r0, r0 + w), at
+//               rec_cur[b] - r0 + slot (rec_cur: b's cursor, starting at
+//               rec_start[b]); the round's records then occupy
+//               [0, w + the largest owner's count).
+enum RecMode { kRecRow = 0, kRecCount = 1, kRecCompact = 2 };
+
+template <bool COUNTS, bool REDO, int REC = kRecRow>
 __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, const int32_t* __restrict__ adj,
                                                  const uint32_t* __restrict__ src, const uint32_t* __restrict__ ndeg,
@@ -87,9 +101,14 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
                                                  uint8_t* __restrict__ flag,
                                                  uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
                                                  uint4* __restrict__ uedge, uint64_t* __restrict__ cnt,
-                                                 uint32_t* __restrict__ ny, uint32_t* __restrict__ nn) {
+                                                 uint32_t* __restrict__ ny, uint32_t* __restrict__ nn,
+                                                 uint32_t* __restrict__ rec_cur = nullptr,
+                                                 const uint32_t* __restrict__ rec_start = nullptr, uint32_t r0 = 0,
+                                                 uint32_t w = 0) {
   constexpr bool redo = REDO;
+  constexpr bool from_flags = REDO || REC == kRecCompact;  // owned entries from the flags already written
   if (redo && cnt[kCntRedo] == 0) return;
+  if (REC == kRecCompact && r0 >= cnt[kCntUEdges]) return;  // a round past the chip's records
   constexpr uint32_t C = kEdgeChunks;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
@@ -124,8 +143,9 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
       if (k < m) {
         const uint32_t x = v[j];
         du[j] -= ub[j];
-        if (redo) {
+        if (from_flags) {
           owned = flag[k] == kFlagUDown;
+          if (REC == kRecCompact && owned) owned = rec_start[row] - r0 < w;  // this round's owners only
         } else {
           uint8_t f = prep_flag(row, x, du[j], dv[j], eps);
           if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
@@ -165,8 +185,12 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
         const uint32_t c = (uint32_t)__popcll(om[j] & seg);
         rank[j] = (uint32_t)__popcll(om[j] & seg & lanemask_lt(lane)) | (h << 24);
         if (head && c) {
-          sb[j] = atomicAdd(&cnt_u[row], c);
-          atomicAdd((unsigned long long*)&work_u[row], (unsigned long long)ws);
+          if (REC == kRecCompact) {
+            sb[j] = atomicAdd(&rec_cur[row], c);  // counts and work came with the first pass
+          } else {
+            sb[j] = atomicAdd(&cnt_u[row], c);
+            atomicAdd((unsigned long long*)&work_u[row], (unsigned long long)ws);
+          }
         }
       }
     }
@@ -175,14 +199,16 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
     // a, d(a))
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
-      if (!om[j]) continue;
+      if (REC == kRecCount || !om[j]) continue;
       const uint32_t slot = __shfl(sb[j], rank[j] >> 24) + (rank[j] & 0xFFFFFFu);
       if ((om[j] >> lane) & 1ull)
-        uedge[ub[j] + slot] = make_uint4((uint32_t)(base + j * kWave + lane),
-                                         mirror ? mirror[base + j * kWave + lane] : kNone, rb[v[j]], dv[j]);
+        uedge[(REC == kRecCompact ? 0u - r0 : ub[j]) + slot] =
+            make_uint4((uint32_t)(base + j * kWave + lane), mirror ? mirror[base + j * kWave + lane] : kNone,
+                       rb[v[j]], dv[j]);
     }
   }
-  block_atomic_add((unsigned long long*)&cnt[redo ? kCntRedoOwned : kCntUEdges], owned_total);
+  if (REC != kRecCompact)
+    block_atomic_add((unsigned long long*)&cnt[redo ? kCntRedoOwned : kCntUEdges], owned_total);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,15 +243,20 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                                  uint4* __restrict__ units, uint32_t ucap,
                                                  uint32_t* __restrict__ tiny, uint64_t* __restrict__ cnt,
                                                  uint8_t* __restrict__ vstate, const uint32_t* __restrict__ ny,
-                                                 const uint32_t* __restrict__ nn, int32_t mu, bool redo) {
+                                                 const uint32_t* __restrict__ nn, int32_t mu, bool redo,
+                                                 const uint32_t* __restrict__ rec_start = nullptr, uint32_t r0 = 0,
+                                                 uint32_t w = 0) {
   constexpr uint32_t V = 4;  // vertices per thread per block iteration
   if (redo && cnt[kCntRedo] == 0) return;
   // Every unit holds at least one owned entry, so ucap units always suffice
   // when each undecided edge is owned once.  An input without a reverse
   // entry for every entry can own more: then no unit is written (the
   // regions from both ends would meet), and the host reports the count.
+  // A round of packed records (rec_start) plans the owners whose records
+  // start in [r0, r0 + w), which hold fewer than ucap records.
   const uint64_t owned_all = cnt[redo ? kCntRedoOwned : kCntUEdges];
-  if (owned_all > ucap) {
+  if (rec_start && r0 >= owned_all) return;
+  if (!rec_start && owned_all > ucap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[kCntOverflow] = owned_all;
     return;
   }
@@ -240,11 +271,13 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
                                       (unsigned long long*)&cnt[kCntTiny]};
   const uint64_t tile = (uint64_t)kBlock * V;
   for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {  // block-uniform
-    uint32_t owned[V], c1[V], c2[V], c3[V], per[V], start[V], stop[V];
+    // records of owner x: [rs, rs + owned) in uedge, its row [start, stop) in adj
+    uint32_t owned[V], c1[V], c2[V], c3[V], per[V], start[V], stop[V], rs[V];
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint64_t x = base + j * kBlock + threadIdx.x;
       owned[j] = x < n ? cnt_u[x] : 0u;
+      if (rec_start && owned[j] && rec_start[x] - r0 >= w) owned[j] = 0;  // another round's owner
     }
     uint32_t need[3] = {0, 0, 0};
 #pragma unroll
@@ -252,10 +285,12 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
       const uint64_t x = base + j * kBlock + threadIdx.x;
       c1[j] = c2[j] = c3[j] = 0;
       per[j] = 1;
-      start[j] = stop[j] = 0;
+      start[j] = stop[j] = rs[j] = 0;
       if (owned[j]) {
-        start[j] = rb[x];  // the owner's edges sit in uedge at its own row range
+        start[j] = rb[x];
         stop[j] = re[x];
+        // the owner's edge records: at its own row range, or packed
+        rs[j] = rec_start ? rec_start[x] - r0 : start[j];
         const uint32_t d = stop[j] - start[j];
         if (tiny_owner(owned[j], work_u[x], d, tune.tiny_work)) {
           c3[j] = 1;  // decided by k_sim_tiny, from the tiny-owner list
@@ -277,9 +312,9 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint32_t cu = c1[j] + c2[j];
-      const uint64_t end = (uint64_t)start[j] + owned[j];
+      const uint64_t end = (uint64_t)rs[j] + owned[j];
       for (uint32_t t = 0; t < cu; ++t) {
-        const uint64_t j0 = (uint64_t)start[j] + (uint64_t)t * per[j];
+        const uint64_t j0 = (uint64_t)rs[j] + (uint64_t)t * per[j];
         const uint64_t j1 = j0 + per[j] < end ? j0 + per[j] : end;
         const uint32_t at = c1[j] ? need[0] + t : ucap - 1 - (need[1] + t);
         units[at] = make_uint4((uint32_t)j0, (uint32_t)j1, start[j], stop[j]);
@@ -295,12 +330,13 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
 // One undecided edge of a tiny owner x (row [bb, be)): merges the short row
 // and the owner row with the same exact threshold and early exit as the
 // tiered kernels, and writes both entries.
-__device__ __forceinline__ void tiny_edge(uint32_t m, uint32_t x, uint32_t bb, uint32_t be, uint32_t i,
-                                          const int32_t* __restrict__ adj, const uint32_t* __restrict__ mirror,
-                                          double eps, const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
+__device__ __forceinline__ void tiny_edge(uint32_t m, uint32_t x, uint32_t bb, uint32_t be, uint32_t rs,
+                                          uint32_t i, const int32_t* __restrict__ adj,
+                                          const uint32_t* __restrict__ mirror, double eps,
+                                          const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
                                           uint32_t* __restrict__ cn_out) {
   const uint32_t l = be - bb;
-  const uint4 ed = uedge[bb + i];  // (entry, reverse entry, row start of a, d(a))
+  const uint4 ed = uedge[rs + i];  // (entry, reverse entry, row start of a, d(a)); rs: x's first record
   const uint32_t k = ed.x, ab = ed.z, s = ed.w, ae = ab + s;
   if (k >= m) return;
   const double rhs = predicate_rhs(eps, s, l);
@@ -370,19 +406,21 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t*
                                                      const uint32_t* __restrict__ cnt_u,
                                                      const uint32_t* __restrict__ tiny,
                                                      const uint4* __restrict__ uedge, uint8_t* __restrict__ flag,
-                                                     uint32_t* __restrict__ cn_out, const uint64_t* __restrict__ cnt) {
-  __shared__ uint32_t s_incl[kBlock], s_x[kBlock], s_bb[kBlock], s_be[kBlock];
+                                                     uint32_t* __restrict__ cn_out, const uint64_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ rec_start, uint32_t r0) {
+  __shared__ uint32_t s_incl[kBlock], s_x[kBlock], s_bb[kBlock], s_be[kBlock], s_rs[kBlock];
   const uint32_t lane = lane_id(), w0 = (threadIdx.x / kWave) * kWave;
   const uint64_t count = cnt[kCntTiny];
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   for (uint64_t g = ((uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * kWave; g < count;
        g += waves * kWave) {
-    uint32_t x = 0, owned = 0, bb = 0, be = 0;
+    uint32_t x = 0, owned = 0, bb = 0, be = 0, rs = 0;
     if (g + lane < count) {
       x = tiny[g + lane];
       owned = cnt_u[x];
       bb = rb[x];
       be = re[x];
+      rs = rec_start ? rec_start[x] - r0 : bb;  // the owner's first edge record
     }
     const uint32_t incl = wave_incl_scan(owned);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
@@ -391,6 +429,7 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t*
     s_x[w0 + lane] = x;
     s_bb[w0 + lane] = bb;
     s_be[w0 + lane] = be;
+    s_rs[w0 + lane] = rs;
     __builtin_amdgcn_wave_barrier();
     for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
       const uint32_t idx = e0 + lane;
@@ -405,7 +444,8 @@ __global__ void __launch_bounds__(kBlock) k_sim_tiny(uint32_t m, const uint32_t*
           lo = mid + 1;
       }
       const uint32_t first = lo ? s_incl[w0 + lo - 1] : 0u;
-      tiny_edge(m, s_x[w0 + lo], s_bb[w0 + lo], s_be[w0 + lo], idx - first, adj, mirror, eps, uedge, flag, cn_out);
+      tiny_edge(m, s_x[w0 + lo], s_bb[w0 + lo], s_be[w0 + lo], s_rs[w0 + lo], idx - first, adj, mirror, eps, uedge,
+                flag, cn_out);
     }
     __builtin_amdgcn_wave_barrier();  // every lane is done with the rows before the next group overwrites them
   }
@@ -1127,6 +1167,12 @@ void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
   k_window_offsets<<<grid_for(total, kBlock), kBlock, 0, s>>>(g->n, g->row_ptr, g->adj, g->wcount, g->woff);
 }
 
+size_t scan_scratch_bytes(uint32_t n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, n);
+  return bytes;
+}
+
 void launch_prep(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s, bool count_all) {
   if (g->m == 0) return;
   const uint32_t grid = grid_for(g->m, kBlock);
@@ -1177,7 +1223,69 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
     const uint64_t want = ((uint64_t)g->n + kBlock - 1) / kBlock, cap = (uint64_t)cus * 32;
     k_sim_tiny<<<(uint32_t)(want < cap ? want : cap), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj,
                                                                      g->mirror, p.eps, b.cnt_u, b.tiny, b.uedge,
-                                                                     b.flag, cn_out, b.cnt);
+                                                                     b.flag, cn_out, b.cnt, nullptr, 0u);
+  }
+}
+
+namespace {
+// Unit, queue and tiny-owner counters before a round of packed records (so a
+// round with no owners runs no unit of the previous one).
+__global__ void k_round_reset(uint64_t* __restrict__ cnt) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[kCntUnits1] = cnt[kCntUnits2] = 0;
+    cnt[kCntQueue1] = cnt[kCntQueue2] = 0;
+    cnt[kCntTiny] = 0;
+  }
+}
+}  // namespace
+
+void launch_prep_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
+  if (g->m == 0) return;
+  k_prep<false, false, kRecCount><<<grid_for(g->m, kBlock), kBlock, 0, s>>>(
+      g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror, p.eps, false, g->pair, b.flag, b.cnt_u,
+      b.work_u, b.uedge, b.cnt, nullptr, nullptr);
+  size_t tmp_bytes = b.scan_tmp_bytes;
+  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tmp_bytes, b.cnt_u, b.rec_start, g->n, s);
+  (void)hipMemcpyAsync(b.rec_cur, b.rec_start, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s);
+}
+
+void launch_similarity_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
+                              hipEvent_t mid, hipStream_t s, uint32_t w, uint32_t rounds) {
+  if (g->m == 0) return;
+  if (impl == kSimWavePerEdge) {  // lists owned entries from the flags; no records
+    launch_similarity_wave(g, b, p, cu_count, s);
+    if (mid) (void)hipEventRecord(mid, s);
+    return;
+  }
+  const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
+  SimTuning t = g->tuning;
+  if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t r0 = r * w;
+    k_round_reset<<<1, kWave, 0, s>>>(b.cnt);
+    k_prep<false, false, kRecCompact><<<grid_for(g->m, kBlock), kBlock, 0, s>>>(
+        g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror, p.eps, false, g->pair, b.flag, b.cnt_u,
+        b.work_u, b.uedge, b.cnt, nullptr, nullptr, b.rec_cur, b.rec_start, r0, w);
+    k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
+                                                     b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, false,
+                                                     b.rec_start, r0, w);
+    k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
+                                                               nullptr, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                               b.ucap, b.cnt, kCntUnits2, kCntQueue2,
+                                                               g->tuning.hash_tables != 0, nullptr, nullptr, nullptr,
+                                                               0);
+    if (r == 0 && mid) (void)hipEventRecord(mid, s);
+    k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
+                                                                   b.flag, nullptr, nullptr, 0, b.uedge, b.units,
+                                                                   b.ucap, b.cnt, kCntUnits1, kCntQueue1,
+                                                                   g->tuning.hash_tables != 0, nullptr, nullptr,
+                                                                   nullptr, 0);
+    if (t.tiny_work) {
+      const uint64_t want = ((uint64_t)g->n + kBlock - 1) / kBlock, cap = (uint64_t)cus * 32;
+      k_sim_tiny<<<(uint32_t)(want < cap ? want : cap), kBlock, 0, s>>>(g->m, g->row_begin, g->row_end, g->adj,
+                                                                       g->mirror, p.eps, b.cnt_u, b.tiny, b.uedge,
+                                                                       b.flag, nullptr, b.cnt, b.rec_start, r0);
+    }
   }
 }
 
