@@ -13,7 +13,7 @@
 //    chip unchanged through its row table (absent rows keep only their
 //    degree).  Block i stays on the device while j advances, so the host
 //    sends about m * (1 + P / 2) entries rather than a copy of every short row
-//    per owner.  Final flags return to a pinned whole-graph flag array; the
+//    per owner.  Final flags return to a whole-graph host flag array; the
 //    next block is staged and the previous chip's flags are merged on the
 //    host while the device works.
 //  * Core check, union-find, border and classify: chips of whole rows with
@@ -22,6 +22,7 @@
 // Labels, core flags and similarity flags are bit-identical to the resident
 // path's.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -182,7 +183,8 @@ struct OocRun {
   uint8_t* h_flag[2] = {nullptr, nullptr};
   uint64_t* h_cnt[2] = {nullptr, nullptr};
   hipEvent_t t0[2] = {nullptr, nullptr}, tm[2] = {nullptr, nullptr}, t1[2] = {nullptr, nullptr};
-  uint8_t* flag = nullptr;  // whole-graph flags, pinned
+  uint8_t* flag = nullptr;  // whole-graph flags, page-locked where the system allows
+  bool flag_pageable = false;
   double bytes_h2d = 0, bytes_d2h = 0, ms_host = 0;
 
   ~OocRun() {
@@ -204,7 +206,8 @@ struct OocRun {
       for (hipEvent_t e : {up_j[i], comp[i], back[i], t0[i], tm[i], t1[i]})
         if (e) (void)hipEventDestroy(e);
     }
-    if (flag) (void)hipHostFree(flag);
+    if (flag && flag_pageable) free(flag);
+    else if (flag) (void)hipHostFree(flag);
     if (dev) (void)hipFree(dev);
   }
 };
@@ -292,12 +295,17 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   pb.scan_tmp_bytes = RLay.scan_bytes;
   R.chip = base + resident;
 
-  const uint64_t stage_entries = std::max<uint64_t>(block_max, 1);
+  // a page-locked graph is sent as it lies: only row offsets are staged
+  const uint64_t stage_entries = g->pinned ? 0 : std::max<uint64_t>(block_max, 1);
+  uint64_t stage_rows = 1;
+  for (const std::vector<uint64_t>* bv : {&blocks, &row_bounds})
+    for (size_t b = 0; b + 1 < bv->size(); ++b) stage_rows = std::max<uint64_t>(stage_rows, (*bv)[b + 1] - (*bv)[b]);
   for (Staging& sg : R.stage) {
-    if (hipHostMalloc((void**)&sg.adj, stage_entries * 4) != hipSuccess ||
-        hipHostMalloc((void**)&sg.off, ((size_t)n + 1) * 4) != hipSuccess) {
+    if ((stage_entries && hipHostMalloc((void**)&sg.adj, stage_entries * 4) != hipSuccess) ||
+        hipHostMalloc((void**)&sg.off, (stage_rows + 1) * 4) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(SGC_ENOMEM, "pinned host staging of %llu bytes failed", (unsigned long long)(stage_entries * 4));
+      return fail(SGC_ENOMEM, "pinned host staging of %llu bytes failed",
+                  (unsigned long long)(stage_entries * 4 + (stage_rows + 1) * 4));
     }
     SGC_HIP(hipEventCreateWithFlags(&sg.free, hipEventDisableTiming));
   }
@@ -317,9 +325,16 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   SGC_HIP(hipEventCreateWithFlags(&R.up_i, hipEventDisableTiming));
   SGC_HIP(hipStreamCreateWithFlags(&R.h2d, hipStreamNonBlocking));
   SGC_HIP(hipStreamCreateWithFlags(&R.d2h, hipStreamNonBlocking));
+  // the whole-graph flags only meet the device through the staged chip
+  // flags and the row chips' uploads: ordinary memory serves when the system
+  // will not lock m more bytes (the row-chip uploads then copy through the
+  // runtime's own staging)
   if (hipHostMalloc((void**)&R.flag, std::max<uint64_t>(g->m, 1)) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(SGC_ENOMEM, "pinned host flag array of %llu bytes failed", (unsigned long long)g->m);
+    R.flag = nullptr;
+    R.flag = (uint8_t*)malloc(std::max<uint64_t>(g->m, 1));
+    if (!R.flag) return fail(SGC_ENOMEM, "host flag array of %llu bytes failed", (unsigned long long)g->m);
+    R.flag_pageable = true;
   }
   SGC_HIP(hipMemcpyAsync(R.deg, deg.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
   R.bytes_h2d += (double)n * 4;

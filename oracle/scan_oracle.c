@@ -370,7 +370,7 @@ uint64_t orc_similarity_entries(uint64_t n, const uint64_t* rp, const int32_t* a
 int orc_labels_from_sim(uint64_t n, const uint64_t* rp, const int32_t* adj, const uint8_t* sim, int32_t mu,
                         int32_t* labels, uint8_t* core, orc_stats* st) {
   const uint64_t m = rp[n];
-  if (m >= (1ull << 32)) return -1;
+  if (n >= (1ull << 31)) return -1;
   orc_stats s;
   memset(&s, 0, sizeof(s));
   s.edges = m / 2;

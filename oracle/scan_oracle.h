@@ -64,8 +64,9 @@ uint64_t orc_similarity_entries(uint64_t n, const uint64_t* row_ptr, const int32
  * by minimum core id (:465-652), borders by the canonical minimum candidate
  * (:654-730), hubs and outliers (:732-798).  Same outputs as orc_scan when
  * sim is orc_scan's own.  For checking a device pass at sizes where
- * orc_scan's sequential similarity is too slow.  stats: cores, clusters,
- * borders, hubs, outliers, similar_edges.  Returns 0, or -1 if m >= 2^32. */
+ * orc_scan's sequential similarity is too slow (any m; 64-bit offsets).
+ * stats: cores, clusters, borders, hubs, outliers, similar_edges.  Returns
+ * 0, or -1 if n >= 2^31. */
 int orc_labels_from_sim(uint64_t n, const uint64_t* row_ptr, const int32_t* adj, const uint8_t* sim,
                         int32_t mu, int32_t* labels, uint8_t* core, orc_stats* stats);
 

@@ -1,0 +1,99 @@
+"""Out-of-core plain SCAN at its configured size (needs an MI355X, about
+150 GB of host memory and several minutes): R-MAT scale 28, edge factor 32,
+seed 1 (n = 268,435,456; m = 16,803,183,218 directed entries, more than
+2^32, so the resident path cannot hold it; about 69 GB of CSR), clustered
+within a 24 GiB device budget at eps 0.3, mu 5.
+
+Checks, as for the resident full-size workloads (tests/test_gpu_configs.py):
+2000 entries on the int32-wrap path (both rows longer than 46338) and a
+uniform sample of 2M entries against the oracle's per-edge similarity; labels and core
+flags against the oracle's phases after similarity over the run's own flag
+array; the degree-only pre-decision count against the oracle's.
+
+It runs only with SGC_C5=1: generating the graph alone takes minutes, beyond
+what the default GPU suite may spend.
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+import efficientstructuralgraphclusteringongpus_amd as sgc
+
+WRAP_ROW = 46338
+NEED_HOST_BYTES = 150 << 30
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def host_available_bytes():
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemAvailable:"):
+                return int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return 0
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(os.environ.get("SGC_C5") != "1",
+                    reason="R-MAT-28 ef32 out of core takes several minutes and ~150 GB of host memory; "
+                           "run with SGC_C5=1")
+def test_rmat28_ef32_out_of_core_24gib(ctx):
+    if host_available_bytes() < NEED_HOST_BYTES:
+        pytest.skip(f"needs {NEED_HOST_BYTES >> 30} GiB of available host memory, "
+                    f"{host_available_bytes() >> 30} GiB here")
+    t0 = time.perf_counter()
+    g = sgc.HostGraph.rmat(28, 32, seed=1)
+    say(f"R-MAT-28 ef32: n={g.n} m={g.m} max degree {g.max_degree}, generated in {time.perf_counter() - t0:.0f} s")
+    assert g.m >= 1 << 32
+    # not page-locked: the graph (69 GB) and the run's own staging together
+    # exceeded what one box would lock; row blocks go through pinned staging
+    eps, mu, budget = 0.3, 5, 24 << 30
+    t0 = time.perf_counter()
+    r, info = sgc.scan_ooc(ctx, g, eps, mu, budget, want_sim=True)
+    wall = time.perf_counter() - t0
+    say(f"out of core: {wall:.1f} s ({g.m // 2 / wall / 1e9:.3f} GTEPS), {info.similarity_chips} similarity chips, "
+        f"{info.row_chips} row chips, {info.bytes_h2d / 1e9:.0f} GB to the device, {info.bytes_d2h / 1e9:.0f} GB back, "
+        f"device {info.device_bytes / 2**30:.2f} GiB, similarity device time {r.stats.ms_similarity / 1e3:.1f} s, "
+        f"host work {info.ms_host / 1e3:.1f} s, later phases {info.ms_phases / 1e3:.1f} s; cores {r.stats.cores}, "
+        f"clusters {r.stats.clusters}")
+    assert info.device_bytes <= budget
+    rp, adj = np.asarray(g.row_ptr), np.asarray(g.adj)
+    # every entry whose edge takes the wrap branch: both rows longer than 46338
+    deg = np.diff(rp)
+    big = np.nonzero(deg > WRAP_ROW)[0]
+    wrap = []
+    for u in big:
+        row = np.arange(rp[u], rp[u + 1], dtype=np.uint64)
+        wrap.append(row[deg[adj[rp[u]:rp[u + 1]]] > WRAP_ROW])
+    wrap = np.concatenate(wrap) if wrap else np.zeros(0, np.uint64)
+    rng = np.random.default_rng(28)
+    n_wrap = len(wrap)
+    if n_wrap > 2000:  # each is a merge of two rows of up to millions of entries for the oracle
+        wrap = rng.choice(wrap, size=2000, replace=False)
+    sample = np.union1d(wrap, rng.integers(0, g.m, size=2_000_000).astype(np.uint64))
+    t0 = time.perf_counter()
+    _, ref = ob.orc_similarity_entries(rp, adj, eps, sample)
+    bad = np.nonzero(r.sim[sample.astype(np.int64)] != ref)[0]
+    say(f"{len(sample)} sampled entries ({len(wrap)} of the {n_wrap} on the wrap path) checked in "
+        f"{time.perf_counter() - t0:.0f} s")
+    assert len(bad) == 0, f"similarity differs at entries {sample[bad[:10]]}"
+    t0 = time.perf_counter()
+    _, _, pu = ob.orc_prep_counts(rp, adj, eps)
+    assert r.stats.u_edges == pu
+    labels, core, st = ob.orc_labels_from_sim(rp, adj, r.sim, mu)
+    say(f"labels recomputed from the flags in {time.perf_counter() - t0:.0f} s")
+    assert np.array_equal(r.core, core), "core flags differ"
+    diff = np.nonzero(r.labels != labels)[0]
+    assert len(diff) == 0, f"labels differ at {diff[:10]}"
+    for k in ("cores", "clusters", "borders", "hubs", "outliers", "similar_edges"):
+        assert getattr(r.stats, k) == st[k], k
