@@ -141,7 +141,7 @@ def host_preprocessing(g, threads):
 
 def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
     """Serving workload: 32 resident R-MAT graphs, n log-uniform in [10k, 5M]
-    (rounded to a power of two: scales 13..22), edge factor uniform in
+    (rounded to a power of two inside that range: scales 14..22), edge factor uniform in
     [8, 32], seeds 1..32; queries (graph uniform, eps uniform on
     {0.1, ..., 0.9}, mu uniform on {2..20}) from a fixed-seed stream.
     Peak qps from a closed-loop run of `queries` queries; p50/p99 latency
@@ -155,7 +155,7 @@ def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
     graphs = []
     for i in range(1, 33):
         n = float(np.exp(rng.uniform(np.log(1e4), np.log(5e6))))
-        scale = int(min(22, max(13, round(np.log2(n)))))
+        scale = int(min(22, max(14, round(np.log2(n)))))  # 2^14 = 16384 >= 10k, 2^22 <= 5M
         ef = int(rng.integers(8, 33))
         graphs.append(sgc.HostGraph.rmat(scale, ef, seed=i))
     gen_s = time.perf_counter() - t0
