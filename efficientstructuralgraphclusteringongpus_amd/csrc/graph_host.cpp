@@ -299,9 +299,11 @@ extern "C" int sgc_graph_load(const char* dir, int check_symmetry, sgc_host_grap
                 (unsigned long long)m);
   struct stat st;
   if (stat(apath.c_str(), &st) != 0) return fail(SGC_EIO, "cannot open %s: %s", apath.c_str(), strerror(errno));
-  if ((uint64_t)st.st_size < m * 4)
-    return fail(SGC_EIO, "%s: %llu bytes, expected %llu", apath.c_str(), (unsigned long long)st.st_size,
-                (unsigned long long)(m * 4));
+  // exactly the degree sum: shorter is a truncated file, longer a malformed one
+  if ((uint64_t)st.st_size != m * 4)
+    return fail((uint64_t)st.st_size < m * 4 ? SGC_EIO : SGC_EFORMAT,
+                "%s: %llu bytes, expected %llu (4 per entry of the degree sum)", apath.c_str(),
+                (unsigned long long)st.st_size, (unsigned long long)(m * 4));
   sgc_host_graph* g = new_host_graph(n, m, false);
   if (!g) return fail(SGC_ENOMEM, "host allocation failed (n=%llu, m=%llu)", (unsigned long long)n, (unsigned long long)m);
   g->row_ptr[0] = 0;

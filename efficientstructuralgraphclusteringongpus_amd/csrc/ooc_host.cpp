@@ -4,7 +4,6 @@
 // blocks, copied in parallel into pinned staging.
 #include "ooc_host.h"
 
-#include <omp.h>
 #include <string.h>
 
 #include <algorithm>
@@ -51,36 +50,6 @@ void ooc_copy_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, uint32_t b
 #pragma omp parallel for schedule(dynamic, 1)
   for (uint64_t p = 0; p < m; p += piece)
     memcpy(dst + p, g->adj + b + p, (size_t)std::min(piece, m - p) * sizeof(int32_t));
-}
-
-void ooc_pair_owned(const sgc_host_graph* g, const std::vector<uint64_t>& bounds, double eps,
-                    std::vector<uint64_t>* out) {
-  const uint64_t* rp = g->row_ptr;
-  const size_t P = bounds.size() - 1;
-  out->assign(P * P, 0);
-  const int T = omp_get_max_threads();
-  std::vector<uint64_t> part((size_t)T * P * P, 0);
-#pragma omp parallel
-  {
-    uint64_t* mine = part.data() + (size_t)omp_get_thread_num() * P * P;
-    // rows in block order: each thread walks whole blocks of rows, so the
-    // owner's block is known without a search
-#pragma omp for schedule(dynamic, 1)
-    for (size_t bi = 0; bi < P; ++bi) {
-      for (uint64_t u = bounds[bi]; u < bounds[bi + 1]; ++u) {
-        const uint32_t du = (uint32_t)(rp[u + 1] - rp[u]);
-        for (uint64_t k = rp[u]; k < rp[u + 1]; ++k) {
-          const uint32_t v = (uint32_t)g->adj[k];
-          const uint32_t dv = (uint32_t)(rp[v + 1] - rp[v]);
-          if (prep_flag((uint32_t)u, v, du, dv, eps) != kFlagUDown) continue;  // u owns its undecided edges
-          const size_t bv = (size_t)(std::upper_bound(bounds.begin(), bounds.end(), (uint64_t)v) - bounds.begin()) - 1;
-          mine[std::min(bi, bv) * P + std::max(bi, bv)] += 1;
-        }
-      }
-    }
-  }
-  for (int t = 0; t < T; ++t)
-    for (size_t q = 0; q < P * P; ++q) (*out)[q] += part[(size_t)t * P * P + q];
 }
 
 void ooc_merge_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, const uint8_t* chip_flag, uint8_t* flag) {

@@ -30,11 +30,4 @@ void ooc_copy_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, uint32_t b
 // their row).
 void ooc_merge_rows(const sgc_host_graph* g, uint64_t r0, uint64_t r1, const uint8_t* chip_flag, uint8_t* flag);
 
-// Undecided edges (the degree-only test of scan_rules.h) whose two rows lie
-// in blocks i <= j of `bounds`, for every pair: out[i * P + j] (P blocks).
-// These are the edges the similarity chip of blocks i and j owns, so the
-// largest entry bounds that chip kind's edge records.
-void ooc_pair_owned(const sgc_host_graph* g, const std::vector<uint64_t>& bounds, double eps,
-                    std::vector<uint64_t>* out);
-
 }  // namespace sgc

@@ -60,6 +60,7 @@ def test_save_writes_reference_layout(tmp_path):
     ("int_size", 3, "int_size"),
     ("m_mismatch", 3, "does not match"),
     ("short_adj", 2, "bytes, expected"),
+    ("long_adj", 3, "bytes, expected"),
     ("self_loop", 3, "Self loop"),
     ("unsorted", 3, "not sorted"),
     ("duplicate", 3, "not sorted"),
@@ -82,6 +83,8 @@ def test_load_rejects(tmp_path, case, code, needle):
         with open(os.path.join(d, "b_degree.bin"), "r+b") as f:
             f.seek(8)
             f.write(struct.pack("<I", 2))
+    elif case == "long_adj":
+        write_raw(d, 2, [1, 1], [1, 0, 1])  # one entry beyond the degree sum
     elif case == "self_loop":
         write_raw(d, 2, [2, 1], [0, 1, 0])
     elif case == "unsorted":
