@@ -33,14 +33,19 @@ __global__ void __launch_bounds__(kBlock) k_fill_src(uint32_t n, const uint32_t*
 // (d(v), v) <= (d(u), u), searches: a binary search of u in the sorted row of
 // v, which writes both mirror[k] and mirror[p].  A pair is found only from
 // entry k, so each word has one writer.  Entries whose reverse entry is
-// missing (asymmetric input) keep the 0xFFFFFFFF the caller fills in first.
-// Four searches per thread run in lockstep so that their loads overlap.
+// missing (asymmetric input) get 0xFFFFFFFF.  Four searches per thread run
+// in lockstep so that their loads overlap; each halves its window only down
+// to eight entries and reads those with eight independent loads, which ends
+// the searches in short rows (most rows) after one round trip.
 __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* __restrict__ rp,
                                                    const uint32_t* __restrict__ src, const int32_t* __restrict__ adj,
                                                    uint32_t* __restrict__ mirror) {
-  constexpr uint32_t C = 4;
+  constexpr uint32_t C = 4;  // searches per thread
+  constexpr uint32_t W = 7;  // windows [lo, hi] of at most W + 1 entries end in one round of loads
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t k0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += stride * C) {
+    // lower bound of key in [lo, end): it lies in [lo, hi], and entries from
+    // hi on are >= key
     uint32_t lo[C], hi[C], end[C];
     int32_t key[C];
     bool on[C];
@@ -61,34 +66,49 @@ __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* _
         }
       }
     }
+    // halve the windows longer than W, all searches in lockstep
     for (;;) {
       bool any = false;
 #pragma unroll
-      for (uint32_t j = 0; j < C; ++j) any |= lo[j] < hi[j];
+      for (uint32_t j = 0; j < C; ++j) any |= hi[j] - lo[j] > W;
       if (!any) break;
       int32_t val[C];
       uint32_t mid[C];
 #pragma unroll
       for (uint32_t j = 0; j < C; ++j) {
         mid[j] = lo[j] + ((hi[j] - lo[j]) >> 1);
-        val[j] = lo[j] < hi[j] ? adj[mid[j]] : 0;
+        val[j] = hi[j] - lo[j] > W ? adj[mid[j]] : 0;
       }
 #pragma unroll
       for (uint32_t j = 0; j < C; ++j)
-        if (lo[j] < hi[j]) {
+        if (hi[j] - lo[j] > W) {
           if (val[j] < key[j])
             lo[j] = mid[j] + 1;
           else
             hi[j] = mid[j];
         }
     }
+    // the last window [lo, hi] (clipped to the row) in one round of loads:
+    // the key's position is lo + (entries below it)
+    int32_t val[C][W + 1];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j)
+#pragma unroll
+      for (uint32_t t = 0; t <= W; ++t)
+        val[j][t] = on[j] && lo[j] + t <= hi[j] && lo[j] + t < end[j] ? adj[lo[j] + t] : 0x7FFFFFFF;
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       if (!on[j]) continue;
-      const uint32_t k = (uint32_t)(k0 + j * stride);
-      const bool found = lo[j] < end[j] && adj[lo[j]] == key[j];
-      mirror[k] = found ? lo[j] : 0xFFFFFFFFu;
-      if (found) mirror[lo[j]] = k;
+      uint32_t below = 0;
+      bool found = false;
+#pragma unroll
+      for (uint32_t t = 0; t <= W; ++t) {
+        below += val[j][t] < key[j];
+        found |= val[j][t] == key[j];
+      }
+      const uint32_t k = (uint32_t)(k0 + j * stride), p = lo[j] + below;
+      mirror[k] = found ? p : 0xFFFFFFFFu;
+      if (found) mirror[p] = k;
     }
   }
 }
