@@ -143,7 +143,7 @@ def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
     """Serving workload: 32 resident R-MAT graphs, n log-uniform in [10k, 5M]
     (rounded to a power of two inside that range: scales 14..22), edge factor uniform in
     [8, 32], seeds 1..32; queries (graph uniform, eps uniform on
-    {0.1, ..., 0.9}, mu uniform on {2..20}) from a fixed-seed stream.
+    {0.10, 0.15, ..., 0.90}, mu uniform on {2..20}) from a fixed-seed stream.
     Peak qps from a closed-loop run of `queries` queries; p50/p99 latency
     (arrival -> labels on the host) under Poisson arrivals at half the peak;
     the one-shot rate runs full plain SCAN passes (labels to the host) for a
@@ -173,7 +173,7 @@ def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
     log(f"serving: 32 graphs, {E} edges, generated in {gen_s:.1f} s, admitted (upload + common-neighbour "
         f"counts) in {admit_s:.1f} s (counting {pre_ms / 1e3:.2f} s)")
     qrng = np.random.default_rng(seed + 1)
-    stream = [(int(qrng.integers(0, 32)), float(qrng.integers(1, 10)) / 10.0, int(qrng.integers(2, 21)))
+    stream = [(int(qrng.integers(0, 32)), float(qrng.integers(2, 19)) / 20.0, int(qrng.integers(2, 21)))
               for _ in range(queries)]
 
     def run(qs, arrivals=None):
@@ -250,7 +250,7 @@ def serving_bench(sgc, ctx, queries: int, seed: int = 2024):
         "edges": E,
         "slots_per_graph": slots,
         "admission_s": round(admit_s, 2),
-        "workload": ("32 R-MAT graphs (scales 13..22, ef 8..32, seeds 1..32), eps uniform on {0.1..0.9}, "
+        "workload": ("32 R-MAT graphs (scales 14..22, ef 8..32, seeds 1..32), eps uniform on {0.10, 0.15, .., 0.90}, "
                      "mu uniform on {2..20}; qps with queries on graphs up to the autotuned cut-off batched (one "
                      "launch per phase over a table of queries), the rest replayed per slot; latency = arrival -> "
                      "labels on the host, Poisson arrivals at half the closed-loop peak"),

@@ -2,9 +2,9 @@
 // chips GPUSCAN++O/Graph.h:70-85, Graph.cpp:414-576; phase-major
 // orchestration SCANoffload.cu:922-1464).
 //
-// Per-vertex state stays resident (degrees, the chip row table, similar-degree
+// Per-vertex state stays resident (the chip row table, similar-degree
 // counts, union-find forest, labels, classify bounds, planner arrays: about
-// 57 bytes per vertex); the rest of the budget holds one chip at a time.
+// 41 bytes per vertex); the rest of the budget holds one chip at a time.
 //  * Similarity: rows are cut into P blocks of at most half a chip.  Chip
 //    (i, j), j >= i, holds blocks i and j and decides exactly the undecided
 //    edges with one endpoint in each (the prep kernel's pair filter marks the
@@ -34,15 +34,6 @@
 
 namespace sgc {
 namespace {
-
-__global__ void __launch_bounds__(kBlock) k_rows_absent(uint32_t n, const uint32_t* __restrict__ deg,
-                                                        uint32_t* __restrict__ rb, uint32_t* __restrict__ re) {
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
-    rb[x] = 0;
-    re[x] = deg[x];
-  }
-}
 
 // Rows r0 .. r0 + nr of a chip: off[] holds their offsets from the block's
 // first entry, `shift` the block's first entry in the chip.
@@ -98,15 +89,13 @@ constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
 struct ResidentLayout {
-  size_t deg, rb, re, off_i, off_j0, off_j1, sd, par, lab, lo, hi, core, cntu, work, cnt, scan, scan_bytes, bytes;
+  size_t rb, re, sd, par, lab, lo, hi, core, cntu, work, cnt, scan, scan_bytes, bytes;
 };
 
 ResidentLayout resident_layout(uint32_t n) {
   Layout L;
   ResidentLayout r;
-  r.deg = L.take((size_t)n * 4), r.rb = L.take((size_t)n * 4), r.re = L.take((size_t)n * 4);
-  r.off_i = L.take(((size_t)n + 1) * 4), r.off_j0 = L.take(((size_t)n + 1) * 4);
-  r.off_j1 = L.take(((size_t)n + 1) * 4);
+  r.rb = L.take((size_t)n * 4), r.re = L.take((size_t)n * 4);
   r.sd = L.take((size_t)n * 4), r.par = L.take((size_t)n * 4), r.lab = L.take((size_t)n * 4);
   r.lo = L.take((size_t)n * 4), r.hi = L.take((size_t)n * 4), r.core = L.take(n);
   r.cntu = L.take((size_t)n * 4), r.work = L.take((size_t)n * 8), r.cnt = L.take(kCntSlots * 8);
@@ -150,11 +139,17 @@ uint64_t record_capacity(uint64_t C, uint64_t maxd) {
   return w + maxd;
 }
 
-size_t row_chip_bytes(uint64_t cap) {
+// Row chip of the later phases: adj, src and flags of `cap` entries, then
+// the row offsets of `rows` rows.
+size_t row_chip_bytes(uint64_t cap, uint64_t rows) {
   Layout L;
-  L.take(cap * 4), L.take(cap * 4), L.take(cap);
+  L.take(cap * 4), L.take(cap * 4), L.take(cap), L.take((rows + 1) * 4);
   return L.bytes;
 }
+
+// Row offsets of block i and the two j buffers for blocks of at most `rows`
+// rows (the row chips use the first of them).
+size_t offsets_bytes(uint64_t rows) { return 3 * (((rows + 1) * 4 + 255) / 256 * 256); }
 
 // Pinned copy of one row block, free again once its upload has run.
 struct Staging {
@@ -169,7 +164,7 @@ struct OocRun {
   const sgc_host_graph* g = nullptr;
   uint32_t n = 0;
   void* dev = nullptr;
-  uint32_t *deg = nullptr, *rb = nullptr, *re = nullptr, *off_i = nullptr, *off_j[2] = {nullptr, nullptr};
+  uint32_t *rb = nullptr, *re = nullptr, *off_i = nullptr, *off_j[2] = {nullptr, nullptr};
   PassBuffers pb{};
   char* chip = nullptr;
   Staging stage[3];  // block i, and one per j buffer
@@ -236,33 +231,46 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
                 (unsigned long long)budget, (unsigned long long)resident, n);
   const uint64_t room = budget - resident - 8192;
   uint64_t cap_block = std::min<uint64_t>(room / kSimBytesPerBlockEntry, kMaxChipEntries / 2);
-  const uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries);
   double t0 = now_ms();
   std::vector<uint32_t> deg(n);
   ooc_degrees(g, deg.data());
   std::vector<uint64_t> blocks, row_bounds;
-  uint64_t block_max = 0, row_max = 0;
+  uint64_t block_max = 0, block_rows = 0, row_max = 0, row_rows = 0;
   uint64_t maxd = 1;
   for (uint32_t x = 0; x < n; ++x) maxd = std::max<uint64_t>(maxd, deg[x]);
   // blocks as large as the room allows once the records of the largest
-  // degree are counted in
+  // degree and the blocks' row offsets are counted in; a block holds at
+  // most as many rows as entries (isolated vertices aside, it holds fewer)
   int rc = SGC_OK;
   uint64_t C = 1, RC = 1;  // entries per block, records per chip at most
   for (;;) {
-    rc = ooc_plan_rows(g, cap_block, &blocks, &block_max);
+    rc = ooc_plan_rows(g, cap_block, cap_block, &blocks, &block_max, &block_rows);
     if (rc) return rc;
     C = std::max<uint64_t>(block_max, 1);
     RC = record_capacity(C, maxd);
-    if (sim_chip_bytes(C, RC) <= room || cap_block <= maxd) break;
+    if (sim_chip_bytes(C, RC) + offsets_bytes(block_rows) <= room || cap_block <= maxd) break;
     cap_block = std::max<uint64_t>(maxd, cap_block - cap_block / 8);
   }
-  rc = ooc_plan_rows(g, cap_row, &row_bounds, &row_max);
-  if (rc) return rc;
+  const size_t sim_need = sim_chip_bytes(C, RC) + offsets_bytes(block_rows);
+  // row chips for the later phases, with their own row offsets, in the same
+  // memory
+  uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries), row_cap = n;
+  for (;;) {
+    rc = ooc_plan_rows(g, std::max<uint64_t>(cap_row, maxd), std::max<uint64_t>(row_cap, 1), &row_bounds, &row_max,
+                       &row_rows);
+    if (rc) return rc;
+    if (row_chip_bytes(std::max<uint64_t>(row_max, 1), row_rows) <= room || cap_row <= maxd) break;
+    cap_row = std::max<uint64_t>(maxd, cap_row - cap_row / 8);
+    row_cap = std::min<uint64_t>(row_cap, cap_row);
+  }
   const uint64_t W = RC - maxd;                // window of one round of owners
   const uint32_t rounds = (uint32_t)((C + W - 1) / W);  // a chip owns at most C edges
   const uint64_t sim_entries = 2 * C;          // entries per chip at most
-  const size_t chip_need = std::max(sim_chip_bytes(C, RC), row_chip_bytes(std::max<uint64_t>(row_max, 1)));
+  const size_t chip_need = std::max(sim_need, row_chip_bytes(std::max<uint64_t>(row_max, 1), row_rows));
   R.ms_host += now_ms() - t0;
+  if (resident + chip_need + 8192 > budget)
+    return fail(SGC_ENOMEM, "device budget %llu bytes is below the %llu bytes this graph needs (sgc_ooc_min_budget)",
+                (unsigned long long)budget, (unsigned long long)(resident + chip_need + 8192));
 
   // ---- device and pinned host memory
   if (hipMalloc(&R.dev, resident + chip_need) != hipSuccess) {
@@ -270,12 +278,14 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     return fail(SGC_ENOMEM, "device allocation of %llu bytes failed", (unsigned long long)(resident + chip_need));
   }
   char* base = (char*)R.dev;
-  R.deg = (uint32_t*)(base + RLay.deg);
   R.rb = (uint32_t*)(base + RLay.rb);
   R.re = (uint32_t*)(base + RLay.re);
-  R.off_i = (uint32_t*)(base + RLay.off_i);
-  R.off_j[0] = (uint32_t*)(base + RLay.off_j0);
-  R.off_j[1] = (uint32_t*)(base + RLay.off_j1);
+  // the similarity chips' three row-offset arrays after their other arrays
+  // (three equal 256-byte-aligned slices)
+  const size_t offs = offsets_bytes(block_rows);
+  R.off_i = (uint32_t*)(base + resident + sim_chip_bytes(C, RC));
+  R.off_j[0] = (uint32_t*)((char*)R.off_i + offs / 3);
+  R.off_j[1] = (uint32_t*)((char*)R.off_i + 2 * (offs / 3));
   PassBuffers& pb = R.pb;
   pb.sd = (uint32_t*)(base + RLay.sd);
   pb.parent = (int32_t*)(base + RLay.par);
@@ -336,8 +346,6 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     if (!R.flag) return fail(SGC_ENOMEM, "host flag array of %llu bytes failed", (unsigned long long)g->m);
     R.flag_pageable = true;
   }
-  SGC_HIP(hipMemcpyAsync(R.deg, deg.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
-  R.bytes_h2d += (double)n * 4;
 
   const ScanParams p{eps, mu};
   uint64_t totals[kCntSlots] = {0};
@@ -434,7 +442,10 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       }
       // flag array b is free once the chip before last has been read back
       if (used[b]) SGC_HIP(hipStreamWaitEvent(s, R.back[b], 0));
-      k_rows_absent<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, R.deg, R.rb, R.re);
+      // rows of the two blocks in this chip's layout; rows outside them are
+      // never read (the prep kernel reads degrees of entries with one end in
+      // each block only, every later kernel rows of the chip's owners and
+      // of their edges' short rows)
       if (nri)
         k_rows_range<<<grid_for(nri, kBlock), kBlock, 0, s>>>((uint32_t)c.r0i, nri, c.j_first ? (uint32_t)c.mj : 0u,
                                                               R.off_i, R.rb, R.re);
@@ -493,10 +504,11 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   int32_t* c_adj = (int32_t*)R.chip;
   Layout RL;
   const size_t ro_adj = RL.take(std::max<uint64_t>(row_max, 1) * 4), ro_src = RL.take(std::max<uint64_t>(row_max, 1) * 4),
-               ro_flag = RL.take(std::max<uint64_t>(row_max, 1));
+               ro_flag = RL.take(std::max<uint64_t>(row_max, 1)), ro_off = RL.take((row_rows + 1) * 4);
   (void)ro_adj;
   uint32_t* c_src = (uint32_t*)(R.chip + ro_src);
   uint8_t* c_flag = (uint8_t*)(R.chip + ro_flag);
+  uint32_t* c_off = (uint32_t*)(R.chip + ro_off);
   sgc_dgraph whole;
   whole.ctx = ctx;
   whole.n = n;
@@ -516,13 +528,13 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     uint32_t* ho = R.stage[0].off;
     SGC_HIP(hipStreamSynchronize(s));  // staging and chip buffers are reused
     for (uint32_t k = 0; k <= nr; ++k) ho[k] = (uint32_t)(g->row_ptr[r0 + k] - b0);
-    SGC_HIP(hipMemcpyAsync(R.off_i, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
+    SGC_HIP(hipMemcpyAsync(c_off, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
     if (mc) {
       SGC_HIP(hipMemcpyAsync(c_adj, g->adj + b0, mc * 4, hipMemcpyHostToDevice, s));
       SGC_HIP(hipMemcpyAsync(c_flag, R.flag + b0, mc, hipMemcpyHostToDevice, s));
     }
     R.bytes_h2d += (double)mc * 5 + (double)(nr + 1) * 4;
-    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, R.off_i, c_src);
+    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, c_off, c_src);
     loaded = i;
     return SGC_OK;
   };
@@ -618,7 +630,8 @@ extern "C" int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t*
   uint64_t maxd = 1;
   for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, g->row_ptr[x + 1] - g->row_ptr[x]);
   // a chip of two blocks that each hold the longest row
-  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd, record_capacity(maxd, maxd)) + 4096;
+  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd, record_capacity(maxd, maxd)) +
+           offsets_bytes(maxd) + 4096;
   return SGC_OK;
 }
 

@@ -18,24 +18,29 @@ void ooc_degrees(const sgc_host_graph* g, uint32_t* deg) {
   for (uint64_t x = 0; x < g->n; ++x) deg[x] = (uint32_t)(rp[x + 1] - rp[x]);
 }
 
-int ooc_plan_rows(const sgc_host_graph* g, uint64_t cap, std::vector<uint64_t>* bounds, uint64_t* max_entries) {
+int ooc_plan_rows(const sgc_host_graph* g, uint64_t cap, uint64_t row_cap, std::vector<uint64_t>* bounds,
+                  uint64_t* max_entries, uint64_t* max_rows) {
   const uint64_t* rp = g->row_ptr;
+  if (row_cap < 1) row_cap = 1;
   bounds->assign(1, 0);
-  uint64_t start = 0, worst = 0;
+  uint64_t start = 0, worst = 0, most = 0;
   for (uint64_t x = 0; x < g->n; ++x) {
     const uint64_t d = rp[x + 1] - rp[x];
     if (d > cap)
       return fail(SGC_ENOMEM, "row %llu has %llu entries, more than a block of the device budget holds (%llu)",
                   (unsigned long long)x, (unsigned long long)d, (unsigned long long)cap);
-    if (rp[x + 1] - rp[start] > cap) {
+    if (rp[x + 1] - rp[start] > cap || x + 1 - start > row_cap) {
       bounds->push_back(x);
       worst = std::max(worst, rp[x] - rp[start]);
+      most = std::max(most, x - start);
       start = x;
     }
   }
   bounds->push_back(g->n);
   worst = std::max(worst, rp[g->n] - rp[start]);
+  most = std::max(most, g->n - start);
   if (max_entries) *max_entries = worst;
+  if (max_rows) *max_rows = most;
   return SGC_OK;
 }
 

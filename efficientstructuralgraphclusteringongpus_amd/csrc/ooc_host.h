@@ -15,9 +15,11 @@ namespace sgc {
 void ooc_degrees(const sgc_host_graph* g, uint32_t* deg);
 
 // Consecutive rows cut into blocks bounds[i] .. bounds[i+1] of at most cap
-// entries each.  *max_entries receives the largest block.  Fails with
+// entries and at most row_cap rows each.  *max_entries and *max_rows receive
+// the largest block's entries and rows (either may be null).  Fails with
 // SGC_ENOMEM if one row alone exceeds cap.
-int ooc_plan_rows(const sgc_host_graph* g, uint64_t cap, std::vector<uint64_t>* bounds, uint64_t* max_entries);
+int ooc_plan_rows(const sgc_host_graph* g, uint64_t cap, uint64_t row_cap, std::vector<uint64_t>* bounds,
+                  uint64_t* max_entries, uint64_t* max_rows);
 
 // Copies the entries of rows [r0, r1) to dst (parallel; skipped for a null
 // dst) and writes their offsets, shifted by `base`, to off[0 .. r1-r0].

@@ -583,6 +583,7 @@ int dispatch(sgc_server* S) {
       memset(&r.st, 0, sizeof(r.st));
       r.st.status = SGC_EINVAL;
       S->done.emplace(it->ticket, std::move(r));
+      S->fresh.push_back(it->ticket);  // listed by sgc_server_completed like any retired query
       it = S->pending.erase(it);
       continue;
     }

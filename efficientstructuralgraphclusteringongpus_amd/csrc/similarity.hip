@@ -118,19 +118,37 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
     // all loads of the C chunks first, then the gathers they feed; with the
     // neighbour-degree index the neighbour's degree is streamed, not gathered
     uint32_t u[C], v[C], ub[C], du[C], dv[C];
+    uint8_t f0[C];  // passes over written flags: the entry's flag
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
       const uint64_t k = base + j * kWave + lane;
       u[j] = k < m ? src[k] : 0xFFFFFFFFu;  // rows must not decrease across lanes
       v[j] = k < m ? (uint32_t)adj[k] : 0u;
       dv[j] = k < m && ndeg ? ndeg[k] : 0u;
+      f0[j] = from_flags && k < m ? flag[k] : kFlagUSkip;
+    }
+    // Only entries that can be owned here need the degree gathers: in a pass
+    // over written flags the owned ones; in an out-of-core chip's first pass
+    // the entries with one endpoint in each of the chip's two blocks (every
+    // other entry is decided in the chip of its own two blocks and is only
+    // marked skipped here).
+    bool need[C];
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+      need[j] = u[j] != 0xFFFFFFFFu;
+      if (from_flags) {
+        need[j] = need[j] && f0[j] == kFlagUDown;
+      } else if (pf.on) {
+        const bool u0 = u[j] - pf.lo0 < pf.hi0 - pf.lo0, u1 = u[j] - pf.lo1 < pf.hi1 - pf.lo1;
+        const bool v0 = v[j] - pf.lo0 < pf.hi0 - pf.lo0, v1 = v[j] - pf.lo1 < pf.hi1 - pf.lo1;
+        need[j] = need[j] && ((u0 && v1) || (u1 && v0));
+      }
     }
 #pragma unroll
     for (uint32_t j = 0; j < C; ++j) {
-      const bool in = u[j] != 0xFFFFFFFFu;
-      ub[j] = in ? rb[u[j]] : 0u;
-      du[j] = in ? re[u[j]] : 0u;
-      if (!ndeg) dv[j] = in ? re[v[j]] - rb[v[j]] : 0u;
+      ub[j] = need[j] ? rb[u[j]] : 0u;
+      du[j] = need[j] ? re[u[j]] : 0u;
+      if (!ndeg) dv[j] = need[j] ? re[v[j]] - rb[v[j]] : 0u;
     }
     uint32_t sb[C], rank[C];
     uint64_t om[C];
@@ -144,16 +162,14 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
         const uint32_t x = v[j];
         du[j] -= ub[j];
         if (from_flags) {
-          owned = flag[k] == kFlagUDown;
+          owned = need[j];
           if (REC == kRecCompact && owned) owned = rec_start[row] - r0 < w;  // this round's owners only
         } else {
-          uint8_t f = prep_flag(row, x, du[j], dv[j], eps);
-          if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
-            f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
-          if (pf.on && f == kFlagUDown) {
-            const bool u0 = row - pf.lo0 < pf.hi0 - pf.lo0, u1 = row - pf.lo1 < pf.hi1 - pf.lo1;
-            const bool v0 = x - pf.lo0 < pf.hi0 - pf.lo0, v1 = x - pf.lo1 < pf.hi1 - pf.lo1;
-            if (!((u0 && v1) || (u1 && v0))) f = kFlagUSkip;
+          uint8_t f = kFlagUSkip;  // outside the chip's pair of blocks
+          if (need[j]) {
+            f = prep_flag(row, x, du[j], dv[j], eps);
+            if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
+              f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
           }
           flag[k] = f;
           owned = f == kFlagUDown;

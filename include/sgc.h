@@ -362,7 +362,8 @@ int sgc_server_autotune_batching(sgc_server* srv, double eps, int32_t mu, uint64
  * (precompute_ms: that pass).  SGC_ENOMEM if the budget cannot hold it; the
  * caller may remove graphs and retry. */
 int sgc_server_add_graph(sgc_server* srv, const sgc_host_graph* g, uint32_t* graph_id, double* precompute_ms);
-/* Waits for the graph's running queries; queued ones end with SGC_EINVAL. */
+/* Waits for the graph's running queries; queued ones end with SGC_EINVAL
+ * (sgc_server_result returns it; sgc_server_completed lists them). */
 int sgc_server_remove_graph(sgc_server* srv, uint32_t graph_id);
 /* Queues a query (eps, mu as sgc_scan) and starts it at once if one of the
  * graph's slots is free. */

@@ -119,9 +119,12 @@ static void check_graph(const sgc_host_graph* g, const std::string& dir) {
   sgc::ooc_degrees(g, deg.data());
   for (uint64_t x = 0; x < n; ++x) CHECK(deg[x] == rp[x + 1] - rp[x]);
   std::vector<uint64_t> bounds;
-  uint64_t most = 0;
-  CHECK(sgc::ooc_plan_rows(g, m / 5 + 1, &bounds, &most) == SGC_OK);
+  uint64_t most = 0, rows = 0;
+  const uint64_t row_cap = n / 7 + 1;
+  CHECK(sgc::ooc_plan_rows(g, m / 5 + 1, row_cap, &bounds, &most, &rows) == SGC_OK);
   CHECK(!bounds.empty() && bounds.front() == 0 && bounds.back() == n);
+  CHECK(rows <= row_cap);
+  for (size_t b = 0; b + 1 < bounds.size(); ++b) CHECK(bounds[b + 1] - bounds[b] <= rows);
   std::vector<uint8_t> flag(m, 0xFF), chip;
   for (size_t b = 0; b + 1 < bounds.size(); ++b) {
     const uint64_t r0 = bounds[b], r1 = bounds[b + 1], e0 = rp[r0], me = rp[r1] - e0;

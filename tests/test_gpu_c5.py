@@ -8,7 +8,11 @@ Checks, as for the resident full-size workloads (tests/test_gpu_configs.py):
 2000 entries on the int32-wrap path (both rows longer than 46338) and a
 uniform sample of 2M entries against the oracle's per-edge similarity; labels and core
 flags against the oracle's phases after similarity over the run's own flag
-array; the degree-only pre-decision count against the oracle's.
+array; the degree-only pre-decision count against the oracle's.  The
+resident path indexes entries with 32 bits and cannot hold this graph, so
+the same pass is run again within a 96 GiB budget (about 9 blocks instead of
+81, so every edge is decided in a different chip): labels, core flags and
+every similarity flag must be identical.
 
 It runs only with SGC_C5=1: generating the graph alone takes minutes, beyond
 what the default GPU suite may spend.
@@ -97,3 +101,11 @@ def test_rmat28_ef32_out_of_core_24gib(ctx):
     assert len(diff) == 0, f"labels differ at {diff[:10]}"
     for k in ("cores", "clusters", "borders", "hubs", "outliers", "similar_edges"):
         assert getattr(r.stats, k) == st[k], k
+    del labels, core
+    t0 = time.perf_counter()
+    r2, info2 = sgc.scan_ooc(ctx, g, eps, mu, 96 << 30, want_sim=True)
+    say(f"96 GiB budget: {time.perf_counter() - t0:.1f} s, {info2.similarity_chips} similarity chips, "
+        f"{info2.bytes_h2d / 1e9:.0f} GB to the device, device {info2.device_bytes / 2**30:.2f} GiB")
+    assert info2.similarity_chips < info.similarity_chips
+    assert np.array_equal(r2.labels, r.labels) and np.array_equal(r2.core, r.core)
+    assert np.array_equal(r2.sim, r.sim), "similarity flags differ between the two budgets"

@@ -122,6 +122,44 @@ def test_admission_and_removal(ctx, graphs):
             srv.submit(gid, 0.4, 2)  # removed graph
 
 
+@pytest.mark.parametrize("batched", [False, True])
+def test_removal_fails_queued_queries(ctx, graphs, batched):
+    """A graph removed with queries still queued: the running ones finish
+    with their results, the queued ones end with an error result (EINVAL)
+    instead of running on freed memory; nothing is left open."""
+    g = graphs[1]
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        ref = {eps: one_shot(ctx, dg, eps, 3).labels for eps in (0.3, 0.5)}
+    finally:
+        dg.close()
+    with sgc.Server(ctx, 4 << 30, slots_per_graph=1) as srv:
+        gid, _ = srv.add_graph(g)
+        if batched:
+            srv.set_batching(1 << 40, 1)
+        # one slot and ~1 ms per query on the GPU: most of these are still
+        # queued when the graph goes
+        tickets = {srv.submit(gid, 0.3 if i % 2 else 0.5, 3): 0.3 if i % 2 else 0.5 for i in range(48)}
+        srv.remove_graph(gid)
+        ok = failed = 0
+        for _ in range(3):
+            srv.progress(wait=False)
+        # every ticket is listed as completed, failed ones included
+        assert set(srv.completed()) == set(tickets)
+        for t, eps in tickets.items():
+            try:
+                r = srv.result(t, g.n)
+            except sgc.SGCError as e:
+                assert sgc.SGCError.CODES[e.code] == "EINVAL" and "removed graph" in e.message
+                failed += 1
+                continue
+            assert r is not None, "a query of a removed graph is still open"
+            assert np.array_equal(r[0], ref[eps])
+            ok += 1
+        say("removed with queries queued", "batched" if batched else "slots", ok, "finished", failed, "failed")
+        assert ok >= 1 and failed >= 1 and srv.pending == 0
+
+
 def test_edgeless_graph_served(ctx):
     g = sgc.HostGraph.from_csr([0, 0, 0, 0], [])
     with sgc.Server(ctx, 1 << 26, slots_per_graph=1) as srv:
