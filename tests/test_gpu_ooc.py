@@ -58,6 +58,24 @@ def test_ooc_matches_resident_rmat(ctx, scale, ef, eps, mu, parts):
     assert_same(r, ref)
 
 
+def test_ooc_many_windows_matches_resident(ctx):
+    """n > 2^20, so neighbour ids span four 2^20-id windows and the
+    workgroup tier makes one pass per window an owner row touches.  Chips
+    have no window-offset index and find where each short row's slice of a
+    pass ends while streaming it; the resident pass reads the slices from
+    the index.  Flags and labels must be identical."""
+    g = sgc.HostGraph.rmat(22, 4, seed=9)
+    say("many windows", g.n, g.m, g.max_degree)
+    assert g.n > 3 << 20 and g.max_degree > 1024
+    for eps, mu in ((0.2, 3), (0.5, 2)):
+        ref = resident(ctx, g, eps, mu)
+        budget = budget_for(g, eps, g.m // 6)
+        r, info = sgc.scan_ooc(ctx, g, eps, mu, budget, want_sim=True)
+        say("chips", info.similarity_chips, eps, mu)
+        assert info.similarity_chips > 1
+        assert_same(r, ref)
+
+
 def test_ooc_from_a_pinned_graph(ctx):
     """Row blocks sent straight from a page-locked graph give the results of
     the staged path, over many similarity chips and row chips."""
