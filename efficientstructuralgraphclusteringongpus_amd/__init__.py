@@ -216,6 +216,7 @@ def load_library(path: str | None = None):
     _sig(lib, "sgc_ctx_destroy", None, [_VP])
     _sig(lib, "sgc_ctx_sync", C.c_int, [_VP])
     _sig(lib, "sgc_ctx_device_name", C.c_char_p, [_VP])
+    _sig(lib, "sgc_ctx_memory", C.c_int, [_VP, _P(C.c_uint64), _P(C.c_uint64)])
     _sig(lib, "sgc_ctx_set_similarity_kernel", C.c_int, [_VP, C.c_int])
     _sig(lib, "sgc_ctx_set_pruning", C.c_int, [_VP, C.c_int])
     _sig(lib, "sgc_dgraph_upload", C.c_int, [_VP, _VP, _P(_VP)])
@@ -434,6 +435,12 @@ class Context:
 
     def sync(self) -> None:
         _check(self._lib.sgc_ctx_sync(self._h))
+
+    def memory(self) -> tuple[int, int]:
+        """(free, total) bytes of device memory."""
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(self._lib.sgc_ctx_memory(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
 
     SIM_TIERED = 0
     SIM_WAVE_PER_EDGE = 1

@@ -182,6 +182,17 @@ extern "C" int sgc_ctx_sync(sgc_ctx* ctx) {
 
 extern "C" const char* sgc_ctx_device_name(const sgc_ctx* ctx) { return ctx ? ctx->name : ""; }
 
+extern "C" int sgc_ctx_memory(sgc_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  clear_error();
+  if (!ctx) return fail(SGC_EINVAL, "null context");
+  SGC_HIP(hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  SGC_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return SGC_OK;
+}
+
 extern "C" int sgc_ctx_set_similarity_kernel(sgc_ctx* ctx, int kind) {
   clear_error();
   if (!ctx) return fail(SGC_EINVAL, "null context");

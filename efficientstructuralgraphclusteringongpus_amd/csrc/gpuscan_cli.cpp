@@ -10,6 +10,11 @@
 //   --text            with --out-dir, also labels.txt (one vertex per line)
 //   --device <id>     HIP device (default 0)
 //   --check-symmetry  also verify that every (u,v) has (v,u)
+//   --device-budget <GiB>  run out of core within that much device memory
+// A graph that the device cannot hold (m >= 2^32 entries, or too little
+// memory for the resident pass) runs out of core within the free device
+// memory less a reserve, as the reference's out-of-core command line does
+// with the same arguments (GPUSCAN++O/main.cpp, SCANoffload.cu:975-978).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -37,6 +42,7 @@ int main(int argc, char* argv[]) {
   int text = 0;
   int device = 0;
   int check_symmetry = 0;
+  double budget_gib = 0;
   for (int i = 4; i < argc; ++i) {
     if (!strcmp(argv[i], "--out") && i + 1 < argc)
       out_path = argv[++i];
@@ -48,7 +54,13 @@ int main(int argc, char* argv[]) {
       device = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--check-symmetry"))
       check_symmetry = 1;
-    else {
+    else if (!strcmp(argv[i], "--device-budget") && i + 1 < argc) {
+      budget_gib = atof(argv[++i]);
+      if (!(budget_gib > 0)) {
+        fprintf(stderr, "--device-budget needs a positive size in GiB\n");
+        return 1;
+      }
+    } else {
       fprintf(stderr, "unknown option %s\n", argv[i]);
       return 1;
     }
@@ -74,16 +86,40 @@ int main(int argc, char* argv[]) {
     fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
     return 1;
   }
-  auto t_mem = high_resolution_clock::now();
-  if (sgc_dgraph_upload(ctx, g, &dg) != SGC_OK) {
-    fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
-    return 1;
-  }
-  fprintf(stderr, "Create GPU Memory time: %.3lf s\n", seconds_since(t_mem));
   std::vector<int32_t> labels(n ? n : 1);
   std::vector<uint8_t> core(n ? n : 1);
   sgc_stats st;
-  if (sgc_scan(ctx, dg, eps, miu, labels.data(), core.data(), nullptr, &st) != SGC_OK) {
+  uint64_t budget = (uint64_t)(budget_gib * (double)(1ull << 30));
+  if (!budget) {
+    auto t_mem = high_resolution_clock::now();
+    const int rc = sgc_dgraph_upload(ctx, g, &dg);
+    if (rc == SGC_ERANGE || rc == SGC_ENOMEM) {
+      // too large for the resident pass: out of core within the free memory
+      uint64_t free_bytes = 0;
+      if (sgc_ctx_memory(ctx, &free_bytes, nullptr) != SGC_OK) {
+        fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+        return 1;
+      }
+      const uint64_t reserve = 256ull << 20;
+      budget = free_bytes > 2 * reserve ? free_bytes - reserve : free_bytes / 2;
+      fprintf(stderr, "graph does not fit the resident pass (%s); out of core\n", sgc_last_error());
+    } else if (rc != SGC_OK) {
+      fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+      return 1;
+    } else {
+      fprintf(stderr, "Create GPU Memory time: %.3lf s\n", seconds_since(t_mem));
+    }
+  }
+  if (budget) {
+    sgc_ooc_stats oi;
+    fprintf(stderr, "out of core within %.2f GiB of device memory\n", (double)budget / (double)(1ull << 30));
+    if (sgc_scan_ooc(ctx, g, eps, miu, budget, labels.data(), core.data(), nullptr, &st, &oi) != SGC_OK) {
+      fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
+      return 1;
+    }
+    fprintf(stderr, "similarity chips: %u, row chips: %u, host to device %.3lf GB, back %.3lf GB\n",
+            oi.similarity_chips, oi.row_chips, oi.bytes_h2d / 1e9, oi.bytes_d2h / 1e9);
+  } else if (sgc_scan(ctx, dg, eps, miu, labels.data(), core.data(), nullptr, &st) != SGC_OK) {
     fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
     return 1;
   }

@@ -143,6 +143,11 @@ void sgc_ctx_destroy(sgc_ctx* ctx);
 int sgc_ctx_sync(sgc_ctx* ctx);
 /* Name of the device the context runs on (e.g. "AMD Instinct MI355X"). */
 const char* sgc_ctx_device_name(const sgc_ctx* ctx);
+/* Free and total device memory in bytes (hipMemGetInfo on the context's
+ * device); either pointer may be NULL.  The reference's out-of-core run
+ * sizes its chips from the free memory in the same way
+ * (GPUSCAN++O/SCANoffload.cu:975-978). */
+int sgc_ctx_memory(sgc_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Common-neighbour kernels of the similarity phase.  TIERED (default): owner
  * rows staged in LDS hash tables, short rows streamed against them.
@@ -282,7 +287,7 @@ int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms);
 /* Out-of-core run details. */
 typedef struct {
   uint64_t device_bytes;     /* device memory the run allocated (<= the budget) */
-  uint32_t similarity_chips; /* owner-row chips of the similarity phase */
+  uint32_t similarity_chips; /* chips (pairs of row blocks) of the similarity phase */
   uint32_t row_chips;        /* row chips of the core / cluster / border / classify passes */
   double bytes_h2d;          /* host-to-device bytes */
   double bytes_d2h;          /* device-to-host bytes */

@@ -32,8 +32,8 @@ def resident(ctx, g, eps, mu):
 
 def budget_for(g, eps, chip_entries):
     """The smallest accepted budget plus room for `chip_entries` more
-    similarity-chip entries (37 bytes each)."""
-    return sgc.ooc_min_budget(g, eps) + 37 * int(chip_entries)
+    similarity-chip block entries (44 bytes each)."""
+    return sgc.ooc_min_budget(g, eps) + 44 * int(chip_entries)
 
 
 def assert_same(a, b):

@@ -311,6 +311,11 @@ def test_reference_entry_point(ctx):
     assert np.array_equal(got, labels)
 
 
+def test_context_memory(ctx):
+    free, total = ctx.memory()
+    assert 0 < free <= total and total >= 1 << 30
+
+
 def test_cli_matches_reference_lines(tmp_path):
     say("GPUScan CLI")
     g = sgc.HostGraph.rmat(13, 16, seed=6)
@@ -336,3 +341,13 @@ def test_cli_matches_reference_lines(tmp_path):
     assert back["summary"]["Cluster Number"] == str(st["clusters"])
     bad = subprocess.run([cli, folder], capture_output=True, text=True)
     assert bad.returncode == 1 and "parameter eps and miu" in bad.stderr
+    # out of core within a small budget (the reference's GPUSCAN++O command
+    # line takes the same arguments): the same labels and count lines
+    budget = max(sgc.ooc_min_budget(g, 0.5) * 1.05, sgc.ooc_min_budget(g, 0.5) + g.m * 11)
+    ooc_out = str(tmp_path / "labels_ooc.bin")
+    proc = subprocess.run([cli, folder, "0.5", "3", "--out", ooc_out, "--device-budget", str(budget / 2**30)],
+                          capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stderr
+    assert "out of core within" in proc.stderr and "similarity chips:" in proc.stderr
+    assert f"core number: {st['cores']}" in proc.stderr and f"Cluster Number: {st['clusters']}" in proc.stderr
+    assert open(ooc_out, "rb").read() == raw
