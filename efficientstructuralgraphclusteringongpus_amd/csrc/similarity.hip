@@ -647,6 +647,20 @@ __device__ __forceinline__ uint32_t count_claim(const ClaimTable& c, uint32_t s0
   for (uint32_t j = 0; j < NL; ++j) x[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + j * kWave) * 4u, 0, 0);
   uint32_t h = 0, used = len;
   if constexpr (CUCKOO) {
+    if constexpr (T::kBitmap) {
+      // a run of windows without exact slices: the claim's elements from
+      // whi on belong to later passes (none of them is a key of this run's
+      // table, so only the consumed count stops there)
+      if (!c.last) {
+        const int32_t whs = (int32_t)c.whi;  // 0 < whi < 2^31 whenever a later window exists
+#pragma unroll
+        for (uint32_t j = 0; j < NL; ++j) {
+          const uint64_t over = __ballot(x[j] >= whs) & load_mask(j, len);
+          if (over && used == len) used = j * kWave + (uint32_t)(__ffsll((long long)over) - 1);
+        }
+        if (used < len && lane == 0) atomicMin(ehi, s0 + used);
+      }
+    }
     if (c.mir_here) record_mirror<NL>(x, len, c.b, s0, mir);
     uint32_t v0[NL], v1[NL];
 #pragma unroll
@@ -748,9 +762,10 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
   uint64_t deferred = 0;  // thread 0: deferred edges of this workgroup
   __shared__ uint4 s_next;  // the next unit's descriptor
   // the owner row's window boundaries, from the window-offset index
-  constexpr uint32_t kMaxOwnerWindows = 64;
+  // (n <= 2^28 for the boundaries in LDS; wstride = windows + 1)
+  constexpr uint32_t kMaxOwnerWindows = 256;
   __shared__ uint32_t s_bw[T::kBitmap ? kMaxOwnerWindows + 1 : 1];
-  const bool bw = T::kBitmap && woff && wstride <= kMaxOwnerWindows + 1;
+  const bool bw_fits = T::kBitmap && wstride > 2 && wstride <= kMaxOwnerWindows + 1;
   const uint32_t lane = lane_id();
   const uint64_t nunits = cnt[units_slot];
   uint64_t streamed = 0, loaded = 0, claims = 0;
@@ -842,17 +857,41 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
       e_mir[t] = k < m ? ed.y : kNone;
       e_state[t] = 0ull;
     }
-    if (bw && threadIdx.x < wstride) s_bw[threadIdx.x] = woff[(size_t)b * wstride + threadIdx.x];
+    // The owner row's window boundaries in LDS: from the window-offset index,
+    // or, without it, by one binary search per boundary in the owner row (rows
+    // longer than a whole-row hash only: the others are one pass)
+    const bool whole = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
+    const bool bw = bw_fits && (woff || !whole);
+    if (bw && threadIdx.x < wstride) {
+      const uint32_t w = threadIdx.x;
+      if (woff) {
+        s_bw[w] = woff[(size_t)b * wstride + w];
+      } else {
+        uint32_t lo = bb, hi = be;
+        if (w + 1 == wstride) {
+          lo = be;
+        } else if (w > 0) {
+          const int32_t key = (int32_t)(w << kWindowBits);
+          while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (adj[mid] < key)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+        }
+        s_bw[w] = lo;
+      }
+    }
     if constexpr (T::kBitmap) __syncthreads();  // every thread picks the passes from s_bw
     SGC_TICK(0);
     // Large tier, per pass over the owner row: the whole row as a cuckoo
     // hash when it has at most kCuckooMaxKeys keys and the ids span more than
-    // one window; else, with the window-offset index, a cuckoo hash of the
-    // longest run of two or more consecutive windows holding at most
-    // kCuckooMaxKeys keys; else the exact bitmap of one window.  A failed
-    // cuckoo build sends the rest of the unit to bitmap windows.
+    // one window; else a cuckoo hash of the longest run of two or more
+    // consecutive windows holding at most kCuckooMaxKeys keys; else the exact
+    // bitmap of one window.  A failed cuckoo build sends the rest of the unit
+    // to bitmap windows.
     bool hash_ok = hash_tables;
-    const bool whole = T::kBitmap && l <= kCuckooMaxKeys && n > (1u << kWindowBits);
     const uint32_t nwin = bw ? wstride - 1 : 0;
     uint32_t kcur = bb;
     while (kcur < be) {
@@ -869,9 +908,13 @@ __global__ void __launch_bounds__(T::kThreads, T::kMinWaves)
           if (bw) {
             // the window of the first key left, from the owner's window
             // boundaries in LDS: the last w with s_bw[w] <= kcur (s_bw[0] is
-            // the row start, so one lane always qualifies)
-            const uint64_t le = __ballot(lane < nwin && s_bw[lane] <= kcur);
-            w0 = 63u - (uint32_t)__clzll((long long)le);
+            // the row start, so one lane always qualifies; the boundaries do
+            // not decrease, so the qualifying windows are a prefix)
+            for (uint32_t base = 0; base < nwin; base += kWave) {
+              const uint64_t le = __ballot(base + lane < nwin && s_bw[base + lane] <= kcur);
+              if (!le) break;
+              w0 = base + 63u - (uint32_t)__clzll((long long)le);
+            }
           } else {
             w0 = (uint32_t)adj[kcur] >> kWindowBits;
           }
@@ -1224,7 +1267,7 @@ void launch_similarity(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, 
   // and holds the long rows; the later tiers decide every edge they own
   const bool prune = p.prune && !cn_out && g->mirror;
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
-                                                             cn_out, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                             cn_out, g->woff, window_count(g->n) + 1, b.uedge, b.units,
                                                              b.ucap, b.cnt, kCntUnits2, kCntQueue2,
                                                              g->tuning.hash_tables != 0, prune ? b.core : nullptr,
                                                              b.ny, b.nn, p.mu);
@@ -1286,7 +1329,7 @@ void launch_similarity_packed(const sgc_dgraph* g, const PassBuffers& b, ScanPar
                                                      b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, false,
                                                      b.rec_start, r0, w);
     k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
-                                                               nullptr, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                               nullptr, g->woff, window_count(g->n) + 1, b.uedge, b.units,
                                                                b.ucap, b.cnt, kCntUnits2, kCntQueue2,
                                                                g->tuning.hash_tables != 0, nullptr, nullptr, nullptr,
                                                                0);
@@ -1342,7 +1385,7 @@ void launch_redo(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu
                                                    b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, true);
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
-                                                             nullptr, g->woff, g->wcount + 1, b.uedge, b.units,
+                                                             nullptr, g->woff, window_count(g->n) + 1, b.uedge, b.units,
                                                              b.ucap, b.cnt, kCntUnits2, kCntQueue2,
                                                              g->tuning.hash_tables != 0, nullptr, nullptr, nullptr, 0);
   k_sim_units<SmallTier><<<cus * 8, SmallTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps,
