@@ -514,8 +514,11 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   whole.n = n;
   whole.m = 0;
   launch_init(&whole, pb, s);
+  // row chip i on the device: rows, flags and entry rows; with_adj also the
+  // neighbour ids (the core count reads flags and rows only)
   size_t loaded = (size_t)-1;
-  auto load_rows = [&](size_t i, sgc_dgraph* v) -> int {
+  bool loaded_adj = false;
+  auto load_rows = [&](size_t i, sgc_dgraph* v, bool with_adj) -> int {
     const uint64_t r0 = row_bounds[i], r1 = row_bounds[i + 1];
     const uint64_t b0 = g->row_ptr[r0], mc = g->row_ptr[r1] - b0;
     v->ctx = ctx;
@@ -523,19 +526,24 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     v->m = (uint32_t)mc;
     v->adj = c_adj;
     v->src = c_src;
-    if (loaded == i) return SGC_OK;
-    const uint32_t nr = (uint32_t)(r1 - r0);
-    uint32_t* ho = R.stage[0].off;
+    if (loaded == i && (loaded_adj || !with_adj)) return SGC_OK;
     SGC_HIP(hipStreamSynchronize(s));  // staging and chip buffers are reused
-    for (uint32_t k = 0; k <= nr; ++k) ho[k] = (uint32_t)(g->row_ptr[r0 + k] - b0);
-    SGC_HIP(hipMemcpyAsync(c_off, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
-    if (mc) {
-      SGC_HIP(hipMemcpyAsync(c_adj, g->adj + b0, mc * 4, hipMemcpyHostToDevice, s));
-      SGC_HIP(hipMemcpyAsync(c_flag, R.flag + b0, mc, hipMemcpyHostToDevice, s));
+    if (loaded != i) {
+      const uint32_t nr = (uint32_t)(r1 - r0);
+      uint32_t* ho = R.stage[0].off;
+      for (uint32_t k = 0; k <= nr; ++k) ho[k] = (uint32_t)(g->row_ptr[r0 + k] - b0);
+      SGC_HIP(hipMemcpyAsync(c_off, ho, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, s));
+      if (mc) SGC_HIP(hipMemcpyAsync(c_flag, R.flag + b0, mc, hipMemcpyHostToDevice, s));
+      R.bytes_h2d += (double)mc + (double)(nr + 1) * 4;
+      if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, c_off, c_src);
+      loaded = i;
+      loaded_adj = false;
     }
-    R.bytes_h2d += (double)mc * 5 + (double)(nr + 1) * 4;
-    if (nr) k_fill_src_rows<<<grid_for(nr, kBlock / kWave), kBlock, 0, s>>>((uint32_t)r0, nr, c_off, c_src);
-    loaded = i;
+    if (with_adj && !loaded_adj) {
+      if (mc) SGC_HIP(hipMemcpyAsync(c_adj, g->adj + b0, mc * 4, hipMemcpyHostToDevice, s));
+      R.bytes_h2d += (double)mc * 4;
+      loaded_adj = true;
+    }
     return SGC_OK;
   };
   PassBuffers rb = pb;
@@ -544,26 +552,33 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   double ms_core = 0, ms_cluster = 0, ms_border = 0, ms_classify = 0;
   SGC_HIP(hipEventRecord(ctx->ev[3], s));
   for (size_t i = 0; i < nrow; ++i) {
-    if ((rc = load_rows(i, &v))) return rc;
+    if ((rc = load_rows(i, &v, false))) return rc;
     launch_core_count(&v, rb, s);
   }
   launch_core_mark(&whole, rb, p, s);
-  progress("ooc: core check issued over %zu row chips", nrow);
+  // without cores there is no core-core edge, no border and no label for
+  // classify to see: the three passes over the row chips would return at
+  // once, so their uploads are skipped (every vertex ends as an outlier, or
+  // as the hub that mu <= 0 makes it, in finalize)
+  SGC_HIP(hipMemcpyAsync(R.h_cnt[0], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
+  SGC_HIP(hipStreamSynchronize(s));
+  const bool any_core = R.h_cnt[0][kCntMarked] != 0;
+  progress("ooc: core check over %zu row chips, %llu cores", nrow, (unsigned long long)R.h_cnt[0][kCntMarked]);
   SGC_HIP(hipEventRecord(ctx->ev[4], s));
-  for (size_t i = 0; i < nrow; ++i) {
-    if ((rc = load_rows(nrow - 1 - i, &v))) return rc;  // last chip first: it is still loaded
+  for (size_t i = 0; any_core && i < nrow; ++i) {
+    if ((rc = load_rows(nrow - 1 - i, &v, true))) return rc;  // last chip first: it is still loaded
     launch_union(&v, rb, s);
   }
   launch_flatten(&whole, rb, s);
   progress("ooc: union-find issued");
   SGC_HIP(hipEventRecord(ctx->ev[5], s));
-  for (size_t i = 0; i < nrow; ++i) {
-    if ((rc = load_rows(i, &v))) return rc;
+  for (size_t i = 0; any_core && i < nrow; ++i) {
+    if ((rc = load_rows(i, &v, true))) return rc;
     launch_border(&v, rb, s);
   }
   SGC_HIP(hipEventRecord(ctx->ev[6], s));
-  for (size_t i = 0; i < nrow; ++i) {
-    if ((rc = load_rows(nrow - 1 - i, &v))) return rc;
+  for (size_t i = 0; any_core && i < nrow; ++i) {
+    if ((rc = load_rows(nrow - 1 - i, &v, true))) return rc;
     launch_classify_edges(&v, rb, s, g->m);
   }
   launch_finalize(&whole, rb, p, s);
