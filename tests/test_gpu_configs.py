@@ -104,3 +104,16 @@ def test_planted_partition_full(ctx):
     st = check_grid(ctx, g, [(0.5, (4,)), (0.08, (4,)), (0.05, (4,))], seed=3)
     assert st[(0.5, 4)]["similar_edges"] == 0
     assert st[(0.05, 4)]["clusters"] > 0 and st[(0.05, 4)]["borders"] > 0
+
+
+def test_rmat24_sparse_without_window_index(ctx):
+    """R-MAT-24 at edge factor 2 (n = 2^24, 16 windows of 2^20 ids, m = 67M):
+    the window-offset index would take n * 17 * 4 = 1.14 GB, more than
+    max(16 m, 1 GiB), so the resident pass runs without it and the
+    workgroup tier finds each long owner row's window boundaries itself
+    (hubs of degree up to 75k span every window).  Sampled flags, labels and
+    core flags against the oracle, as above."""
+    g = sgc.HostGraph.rmat(24, 2, seed=3)
+    say(f"R-MAT-24 ef2: n={g.n} m={g.m} max degree {g.max_degree}")
+    assert g.n * 17 * 4 > max(16 * g.m, 1 << 30) and g.max_degree > 8192
+    check_grid(ctx, g, [(0.2, (2,)), (0.5, (3,))], seed=2)
