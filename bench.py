@@ -108,6 +108,36 @@ def cpu_baseline(g, eps_list, seconds: float):
     }
 
 
+def c1_full_pass(sgc, ctx):
+    """The CPU-runnable workload end to end: R-MAT-16 (edge factor 16, seed 1),
+    eps 0.5, mu 5.  The oracle's whole pass (degree-only prep, every
+    similarity, core check, union-find, borders, hubs and outliers, one
+    thread, the reference's sequential interleaving) beside the GPU pass on
+    the resident graph; labels and core flags must be identical."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    g = sgc.HostGraph.rmat(16, 16, seed=1)
+    t0 = time.perf_counter()
+    labels, core, _, st = ob.orc_scan(g.row_ptr, g.adj, 0.5, 5)
+    cpu_s = time.perf_counter() - t0
+    dg = sgc.DeviceGraph(ctx, g)
+    try:
+        sgc.scan(ctx, dg, 0.5, 5)
+        r = sgc.scan(ctx, dg, 0.5, 5)
+    finally:
+        dg.close()
+    same = bool(np.array_equal(r.labels, labels) and np.array_equal(r.core, core))
+    e = g.m // 2
+    log(f"C1 R-MAT-16 eps 0.5 mu 5: oracle whole pass {cpu_s * 1e3:.1f} ms (1 thread), GPU pass "
+        f"{r.stats.ms_total:.3f} ms, identical labels {same}")
+    return {"workload": "R-MAT scale 16 ef 16 seed 1, eps 0.5 mu 5", "edges": e, "cpu_whole_pass_ms": round(cpu_s * 1e3, 2),
+            "cpu_threads": 1, "cpu_gteps": e / cpu_s / 1e9, "gpu_pass_ms": round(r.stats.ms_total, 4),
+            "gpu_gteps": e / (r.stats.ms_total * 1e-3) / 1e9, "labels_identical": same, "clusters": st["clusters"]}
+
+
 def host_preprocessing(g, threads):
     """The reference's host-side preparation, timed on the same cores and kept
     out of every rate: reading b_degree.bin / b_adj.bin (GPUSCAN++/Graph.cpp:
@@ -520,6 +550,7 @@ def main() -> int:
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(g, eps_list, args.cpu_seconds)
+        cpu["c1_whole_pass"] = c1_full_pass(sgc, ctx)
 
     out = {
         "metric": METRIC,
