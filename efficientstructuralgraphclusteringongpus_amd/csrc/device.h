@@ -197,6 +197,10 @@ void launch_neighbor_degrees(const sgc_dgraph* g, hipStream_t s);
 // 2^20-id windows of the large-tier bitmap covering n ids, and the kernel
 // filling g->woff (see sgc_dgraph).
 uint32_t window_count(uint32_t n);
+// Bytes of the window-offset index a resident graph of n vertices and m
+// entries keeps: n * (W + 1) words when there is more than one window and
+// that is at most max(16 m, 1 GiB) bytes (four times the adjacency); else 0.
+uint64_t window_index_bytes(uint64_t n, uint64_t m);
 void launch_window_offsets(const sgc_dgraph* g, hipStream_t s);
 void launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s);

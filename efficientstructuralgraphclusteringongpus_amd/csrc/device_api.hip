@@ -262,11 +262,10 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   // most four times the adjacency's bytes (it grows with n^2: R-MAT-26 at
   // edge factor 16 needs 17 GB, twice its adjacency, and a pass there runs
   // 2.5 times faster with it)
-  const uint32_t W = window_count(d->n);
-  const uint64_t woff_bytes = (uint64_t)d->n * (W + 1) * 4;
-  if (W > 1 && woff_bytes <= std::max<uint64_t>(4 * m_bytes, 1ull << 30)) {
+  const uint64_t woff_bytes = window_index_bytes(d->n, g->m);
+  if (woff_bytes) {
     if (hipMalloc(&d->woff, woff_bytes) == hipSuccess) {
-      d->wcount = W;
+      d->wcount = window_count(d->n);
     } else {
       (void)hipGetLastError();
       d->woff = nullptr;  // the kernels find window boundaries while streaming instead

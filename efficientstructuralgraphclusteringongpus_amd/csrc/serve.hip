@@ -723,10 +723,8 @@ extern "C" uint64_t sgc_server_graph_bytes(const sgc_host_graph* g, uint32_t slo
   // The admission pass's similarity scratch is the context's and is not
   // held per graph.
   const uint64_t m_bytes = align256(std::max<uint64_t>(m, 1) * 4);
-  const uint32_t W = window_count((uint32_t)n);
-  const uint64_t woff = (uint64_t)n * (W + 1) * 4;
-  const uint64_t graph = align256((n + 1) * 4) + 5 * m_bytes +
-                         (W > 1 && woff <= std::max<uint64_t>(m_bytes, 1ull << 30) ? align256(woff) : 0);
+  const uint64_t woff = window_index_bytes(n, m);
+  const uint64_t graph = align256((n + 1) * 4) + 5 * m_bytes + (woff ? align256(woff) : 0);
   return graph + slots_per_graph * slot_bytes(n, m);
 }
 

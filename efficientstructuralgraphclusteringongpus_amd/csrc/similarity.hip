@@ -23,6 +23,8 @@
 //    window at a time as an exact 128 KB bitmap.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <hipcub/device/device_scan.hpp>
 
 #include "device.h"
@@ -1219,6 +1221,12 @@ __global__ void __launch_bounds__(kBlock) k_window_offsets(uint32_t n, const uin
 }  // namespace
 
 uint32_t window_count(uint32_t n) { return (uint32_t)(((uint64_t)n + (1u << kWindowBits) - 1) >> kWindowBits); }
+
+uint64_t window_index_bytes(uint64_t n, uint64_t m) {
+  const uint32_t W = window_count((uint32_t)n);
+  const uint64_t bytes = n * (W + 1) * 4;
+  return W > 1 && bytes <= std::max<uint64_t>(16 * std::max<uint64_t>(m, 1), 1ull << 30) ? bytes : 0;
+}
 
 void launch_window_offsets(const sgc_dgraph* g, hipStream_t s) {
   if (!g->woff || g->n == 0) return;
