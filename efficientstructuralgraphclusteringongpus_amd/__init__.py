@@ -68,6 +68,7 @@ class _OocStats(C.Structure):
     _fields_ = [
         ("device_bytes", C.c_uint64), ("similarity_chips", C.c_uint32), ("row_chips", C.c_uint32),
         ("bytes_h2d", C.c_double), ("bytes_d2h", C.c_double), ("ms_host", C.c_double), ("ms_phases", C.c_double),
+        ("row_blocks", C.c_uint32), ("blocks_per_group", C.c_uint32),
     ]
 
 
@@ -80,6 +81,8 @@ class OocStats:
     bytes_d2h: float
     ms_host: float
     ms_phases: float
+    row_blocks: int
+    blocks_per_group: int
 
 
 class _QueryStats(C.Structure):

@@ -89,9 +89,13 @@ namespace sgc {
 // Out-of-core chip of two row blocks [lo0, hi0) and [lo1, hi1) (equal for a
 // chip of one block): the prep kernel keeps only owned edges whose endpoints
 // lie one in each block.
+// keep0: entries of rows in [lo0, hi0) outside the pair keep their flag (an
+// earlier chip of the same group of blocks decided them); every other entry
+// outside the pair is marked kFlagUSkip.
 struct PairFilter {
   uint32_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
   bool on = false;
+  bool keep0 = false;
 };
 }  // namespace sgc
 

@@ -117,8 +117,11 @@ int main(int argc, char* argv[]) {
       fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
       return 1;
     }
-    fprintf(stderr, "similarity chips: %u, row chips: %u, host to device %.3lf GB, back %.3lf GB\n",
-            oi.similarity_chips, oi.row_chips, oi.bytes_h2d / 1e9, oi.bytes_d2h / 1e9);
+    fprintf(stderr,
+            "similarity chips: %u (%u row blocks in groups of %u), row chips: %u, host to device %.3lf GB, back "
+            "%.3lf GB\n",
+            oi.similarity_chips, oi.row_blocks, oi.blocks_per_group, oi.row_chips, oi.bytes_h2d / 1e9,
+            oi.bytes_d2h / 1e9);
   } else if (sgc_scan(ctx, dg, eps, miu, labels.data(), core.data(), nullptr, &st) != SGC_OK) {
     fprintf(stderr, "GPUassert: %s\n", sgc_last_error());
     return 1;

@@ -5,17 +5,20 @@
 // Per-vertex state stays resident (the chip row table, similar-degree
 // counts, union-find forest, labels, classify bounds, planner arrays: about
 // 41 bytes per vertex); the rest of the budget holds one chip at a time.
-//  * Similarity: rows are cut into P blocks of at most half a chip.  Chip
-//    (i, j), j >= i, holds blocks i and j and decides exactly the undecided
-//    edges with one endpoint in each (the prep kernel's pair filter marks the
-//    other owned edges kFlagUSkip); every edge has both of its rows in exactly
-//    one chip.  The resident prep, planner and similarity kernels run on the
-//    chip unchanged through its row table (absent rows keep only their
-//    degree).  Block i stays on the device while j advances, so the host
-//    sends about m * (1 + P / 2) entries rather than a copy of every short row
-//    per owner.  Final flags return to a whole-graph host flag array; the
-//    next block is staged and the previous chip's flags are merged on the
-//    host while the device works.
+//  * Similarity: rows are cut into P blocks, and consecutive blocks into
+//    groups of k (4 where the budget allows).  A group's own chip decides the
+//    undecided edges with both endpoints in the group; chip (group, j), for
+//    every block j after the group, those with one endpoint in each (the
+//    prep kernel's pair filter marks the other entries kFlagUSkip, or keeps
+//    the group's flags from its earlier chips); every edge has both of its
+//    rows in exactly one chip.  The resident prep, planner and similarity
+//    kernels run on the chip unchanged through its row table (rows outside
+//    the chip are never read).  The group stays on the device while j
+//    advances, so the host sends about m * (1 + P / (2k)) entries rather
+//    than a copy of every short row per owner.  Final flags return to a
+//    whole-graph host flag array (a block's after its chip, a group's after
+//    its last); the next block is staged and the previous chip's flags are
+//    merged on the host while the device works.
 //  * Core check, union-find, border and classify: chips of whole rows with
 //    their flags, one pass per phase (each phase needs the previous one's
 //    global result); when all rows fit in one chip it is uploaded once.
@@ -73,18 +76,24 @@ struct Layout {
   }
 };
 
-// Similarity chips are two row blocks of at most C entries.  Their device
-// memory: adj and src as [ j-buffer 0 | block i | j-buffer 1 ] (3C entries
-// each; the second block of a chip sits right before the first in buffer 0
-// or right after it in buffer 1, so every chip is one contiguous entry range
-// and the next chip's second block uploads while this one runs), two flag
-// arrays of 2C (consecutive chips alternate, so one chip's flags go back to
-// the host while the next one computes), and R packed edge records and R + 2
-// work units (16 B each) for one round of owners.  A chip owns at most C
-// undecided edges (each has both entries in the chip), decided in rounds of
-// owners whose records start in a window of W = R - (largest degree), so
-// R = C / 2 needs two or three rounds: 28 + 32 R / C = 44 B per block entry.
-constexpr uint64_t kSimBytesPerBlockEntry = 3 * 4 + 3 * 4 + 2 * 2 + 16;  // with R = C / 2
+// Similarity chips pair a group of up to k consecutive row blocks (I
+// entries at most) with one later row block (J entries at most), or hold a
+// group alone.  Their device memory: adj, src and flags as [ j-buffer 0 |
+// group | j-buffer 1 ] (I + 2J entries each; the row block sits right
+// before the group in buffer 0 or right after it in buffer 1, so every chip
+// is one contiguous entry range and the next chip's row block uploads while
+// this one runs), and R packed edge records and R + 2 work units (16 B each)
+// for one round of owners.  The group's flags stay on the device across its
+// chips (a chip writes there only the entries it decides) and go back to the
+// host once; a row block's flags go back after its chip, while the next chip
+// computes in the other buffer.  A chip owns at most (I + J) / 2 undecided
+// edges, decided in rounds of owners whose records start in a window of
+// W = R - (largest degree), with R about half of that: two or three rounds.
+// Per row-block entry a chip takes about 17 k + 26 bytes for groups of k
+// blocks, and the host sends about m (1 + P / (2k)) entries for P row
+// blocks, so for the same memory groups of 4 send about half of what pairs of
+// single blocks send.
+constexpr uint32_t kGroupChoices[] = {4, 2, 1};  // the largest the room allows
 constexpr uint64_t kRowChipBytesPerEntry = 4 + 4 + 1;
 constexpr uint64_t kMaxChipEntries = 0xFFFFFFF0ull;
 
@@ -106,34 +115,32 @@ ResidentLayout resident_layout(uint32_t n) {
 }
 
 struct SimChipArrays {
-  int32_t* adj;      // 3C
-  uint32_t* src;     // 3C
-  uint8_t* flag[2];  // 2C each
-  uint4* uedge;      // R
-  uint4* units;      // R + 2
+  int32_t* adj;   // I + 2J
+  uint32_t* src;  // I + 2J
+  uint8_t* flag;  // I + 2J
+  uint4* uedge;   // R
+  uint4* units;   // R + 2
 };
 
-SimChipArrays sim_arrays(char* base, uint64_t C, uint64_t R, size_t* bytes) {
+SimChipArrays sim_arrays(char* base, uint64_t I, uint64_t J, uint64_t R, size_t* bytes) {
   Layout L;
-  const size_t o_adj = L.take(3 * C * 4), o_src = L.take(3 * C * 4), o_f0 = L.take(2 * C), o_f1 = L.take(2 * C),
+  const uint64_t e = I + 2 * J;
+  const size_t o_adj = L.take(e * 4), o_src = L.take(e * 4), o_flag = L.take(e),
                o_uedge = L.take(R * sizeof(uint4)), o_units = L.take((R + 2) * sizeof(uint4));
   if (bytes) *bytes = L.bytes;
-  return SimChipArrays{(int32_t*)(base + o_adj),
-                       (uint32_t*)(base + o_src),
-                       {(uint8_t*)(base + o_f0), (uint8_t*)(base + o_f1)},
-                       (uint4*)(base + o_uedge),
-                       (uint4*)(base + o_units)};
+  return SimChipArrays{(int32_t*)(base + o_adj), (uint32_t*)(base + o_src), (uint8_t*)(base + o_flag),
+                       (uint4*)(base + o_uedge), (uint4*)(base + o_units)};
 }
 
-size_t sim_chip_bytes(uint64_t C, uint64_t R) {
+size_t sim_chip_bytes(uint64_t I, uint64_t J, uint64_t R) {
   size_t b = 0;
-  sim_arrays(nullptr, C, R, &b);
+  sim_arrays(nullptr, I, J, R, &b);
   return b;
 }
 
-// Records per chip for blocks of C entries and rows of at most maxd entries:
-// a window of at least C / 2 - maxd (and at least C / 4) plus the largest
-// owner's count.
+// Records per chip for chips owning at most C edges and rows of at most
+// maxd entries: a window of at least C / 2 - maxd (and at least C / 4) plus
+// the largest owner's count.
 uint64_t record_capacity(uint64_t C, uint64_t maxd) {
   const uint64_t half = C / 2, w = half > maxd + C / 4 ? half - maxd : std::max<uint64_t>(C / 4, 1);
   return w + maxd;
@@ -147,9 +154,10 @@ size_t row_chip_bytes(uint64_t cap, uint64_t rows) {
   return L.bytes;
 }
 
-// Row offsets of block i and the two j buffers for blocks of at most `rows`
-// rows (the row chips use the first of them).
-size_t offsets_bytes(uint64_t rows) { return 3 * (((rows + 1) * 4 + 255) / 256 * 256); }
+// Row offsets of a group of rows_i rows and of the two j buffers of rows_j
+// rows each, 256-byte aligned.
+size_t offsets_slice(uint64_t rows) { return ((rows + 1) * 4 + 255) / 256 * 256; }
+size_t offsets_bytes(uint64_t rows_i, uint64_t rows_j) { return offsets_slice(rows_i) + 2 * offsets_slice(rows_j); }
 
 // Pinned copy of one row block, free again once its upload has run.
 struct Staging {
@@ -174,8 +182,10 @@ struct OocRun {
   hipStream_t h2d = nullptr, d2h = nullptr;
   hipEvent_t up_i = nullptr, up_j[2] = {nullptr, nullptr};  // block uploads done
   hipEvent_t comp[2] = {nullptr, nullptr};                  // the chip of buffer b done on the device
-  hipEvent_t back[2] = {nullptr, nullptr};                  // its flags back on the host
-  uint8_t* h_flag[2] = {nullptr, nullptr};
+  hipEvent_t back[2] = {nullptr, nullptr};                  // its row block's flags back on the host
+  hipEvent_t back_i = nullptr;                              // the last group's flags back on the host
+  uint8_t* h_flag[2] = {nullptr, nullptr};                  // a row block's flags (J)
+  uint8_t* h_iflag = nullptr;                               // a group's flags (I)
   uint64_t* h_cnt[2] = {nullptr, nullptr};
   hipEvent_t t0[2] = {nullptr, nullptr}, tm[2] = {nullptr, nullptr}, t1[2] = {nullptr, nullptr};
   uint8_t* flag = nullptr;  // whole-graph flags, page-locked where the system allows
@@ -195,6 +205,8 @@ struct OocRun {
       if (st.free) (void)hipEventDestroy(st.free);
     }
     if (up_i) (void)hipEventDestroy(up_i);
+    if (back_i) (void)hipEventDestroy(back_i);
+    if (h_iflag) (void)hipHostFree(h_iflag);
     for (int i = 0; i < 2; ++i) {
       if (h_flag[i]) (void)hipHostFree(h_flag[i]);
       if (h_cnt[i]) (void)hipHostFree(h_cnt[i]);
@@ -230,28 +242,46 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     return fail(SGC_ENOMEM, "device budget %llu bytes is below the %llu resident bytes the graph's %u vertices need",
                 (unsigned long long)budget, (unsigned long long)resident, n);
   const uint64_t room = budget - resident - 8192;
-  uint64_t cap_block = std::min<uint64_t>(room / kSimBytesPerBlockEntry, kMaxChipEntries / 2);
   double t0 = now_ms();
   std::vector<uint32_t> deg(n);
   ooc_degrees(g, deg.data());
   std::vector<uint64_t> blocks, row_bounds;
-  uint64_t block_max = 0, block_rows = 0, row_max = 0, row_rows = 0;
+  uint64_t block_rows = 0, row_max = 0, row_rows = 0;
   uint64_t maxd = 1;
   for (uint32_t x = 0; x < n; ++x) maxd = std::max<uint64_t>(maxd, deg[x]);
-  // blocks as large as the room allows once the records of the largest
-  // degree and the blocks' row offsets are counted in; a block holds at
-  // most as many rows as entries (isolated vertices aside, it holds fewer)
+  // Row blocks as large as the room allows once the records of the largest
+  // degree and the row offsets are counted in (a block holds at most as many
+  // rows as entries), in groups of the largest size whose chips still fit
+  // with blocks of the longest row.
   int rc = SGC_OK;
-  uint64_t C = 1, RC = 1;  // entries per block, records per chip at most
-  for (;;) {
-    rc = ooc_plan_rows(g, cap_block, cap_block, &blocks, &block_max, &block_rows);
-    if (rc) return rc;
-    C = std::max<uint64_t>(block_max, 1);
-    RC = record_capacity(C, maxd);
-    if (sim_chip_bytes(C, RC) + offsets_bytes(block_rows) <= room || cap_block <= maxd) break;
-    cap_block = std::max<uint64_t>(maxd, cap_block - cap_block / 8);
+  uint64_t I = 1, J = 1, RC = 1, rows_i = 0;  // group and block entries, records per round, group rows
+  uint32_t group = 1;
+  bool fits = false;
+  for (const uint32_t k : kGroupChoices) {
+    uint64_t cap = std::max<uint64_t>(std::min<uint64_t>(room / (17 * k + 26), kMaxChipEntries / (k + 2)), maxd);
+    for (;;) {
+      uint64_t block_max = 0;
+      rc = ooc_plan_rows(g, cap, cap, &blocks, &block_max, &block_rows);
+      if (rc) return rc;
+      J = std::max<uint64_t>(block_max, 1);
+      I = 1, rows_i = 0;
+      for (size_t b = 0; b + 1 < blocks.size(); b += k) {
+        const size_t e = std::min<size_t>(b + k, blocks.size() - 1);
+        I = std::max<uint64_t>(I, g->row_ptr[blocks[e]] - g->row_ptr[blocks[b]]);
+        rows_i = std::max<uint64_t>(rows_i, blocks[e] - blocks[b]);
+      }
+      RC = record_capacity((I + J) / 2 + 1, maxd);
+      if (I + 2 * J <= kMaxChipEntries && sim_chip_bytes(I, J, RC) + offsets_bytes(rows_i, block_rows) <= room) {
+        fits = true;
+        break;
+      }
+      if (cap <= maxd) break;
+      cap = std::max<uint64_t>(maxd, cap - cap / 8);
+    }
+    group = k;
+    if (fits) break;
   }
-  const size_t sim_need = sim_chip_bytes(C, RC) + offsets_bytes(block_rows);
+  const size_t sim_need = sim_chip_bytes(I, J, RC) + offsets_bytes(rows_i, block_rows);
   // row chips for the later phases, with their own row offsets, in the same
   // memory
   uint64_t cap_row = std::min<uint64_t>(room / kRowChipBytesPerEntry, kMaxChipEntries), row_cap = n;
@@ -263,9 +293,10 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     cap_row = std::max<uint64_t>(maxd, cap_row - cap_row / 8);
     row_cap = std::min<uint64_t>(row_cap, cap_row);
   }
-  const uint64_t W = RC - maxd;                // window of one round of owners
-  const uint32_t rounds = (uint32_t)((C + W - 1) / W);  // a chip owns at most C edges
-  const uint64_t sim_entries = 2 * C;          // entries per chip at most
+  const uint64_t W = RC - maxd;                          // window of one round of owners
+  const uint64_t owned_max = (I + J) / 2 + 1;               // undecided edges one chip owns at most
+  const uint32_t rounds = (uint32_t)((owned_max + W - 1) / W);
+  const uint64_t sim_entries = I + J;                       // entries per chip at most
   const size_t chip_need = std::max(sim_need, row_chip_bytes(std::max<uint64_t>(row_max, 1), row_rows));
   R.ms_host += now_ms() - t0;
   if (resident + chip_need + 8192 > budget)
@@ -281,11 +312,9 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   R.rb = (uint32_t*)(base + RLay.rb);
   R.re = (uint32_t*)(base + RLay.re);
   // the similarity chips' three row-offset arrays after their other arrays
-  // (three equal 256-byte-aligned slices)
-  const size_t offs = offsets_bytes(block_rows);
-  R.off_i = (uint32_t*)(base + resident + sim_chip_bytes(C, RC));
-  R.off_j[0] = (uint32_t*)((char*)R.off_i + offs / 3);
-  R.off_j[1] = (uint32_t*)((char*)R.off_i + 2 * (offs / 3));
+  R.off_i = (uint32_t*)(base + resident + sim_chip_bytes(I, J, RC));
+  R.off_j[0] = (uint32_t*)((char*)R.off_i + offsets_slice(rows_i));
+  R.off_j[1] = (uint32_t*)((char*)R.off_j[0] + offsets_slice(block_rows));
   PassBuffers& pb = R.pb;
   pb.sd = (uint32_t*)(base + RLay.sd);
   pb.parent = (int32_t*)(base + RLay.par);
@@ -305,25 +334,31 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   pb.scan_tmp_bytes = RLay.scan_bytes;
   R.chip = base + resident;
 
-  // a page-locked graph is sent as it lies: only row offsets are staged
-  const uint64_t stage_entries = g->pinned ? 0 : std::max<uint64_t>(block_max, 1);
-  uint64_t stage_rows = 1;
-  for (const std::vector<uint64_t>* bv : {&blocks, &row_bounds})
-    for (size_t b = 0; b + 1 < bv->size(); ++b) stage_rows = std::max<uint64_t>(stage_rows, (*bv)[b + 1] - (*bv)[b]);
-  for (Staging& sg : R.stage) {
-    if ((stage_entries && hipHostMalloc((void**)&sg.adj, stage_entries * 4) != hipSuccess) ||
-        hipHostMalloc((void**)&sg.off, (stage_rows + 1) * 4) != hipSuccess) {
+  // a page-locked graph is sent as it lies: only row offsets are staged.
+  // Staging 0 holds a group (and the row chips' offsets), 1 and 2 a row
+  // block each.
+  for (int q = 0; q < 3; ++q) {
+    Staging& sg = R.stage[q];
+    const uint64_t entries = g->pinned ? 0 : std::max<uint64_t>(q ? J : I, 1);
+    const uint64_t rows = std::max<uint64_t>(q ? block_rows : std::max(rows_i, row_rows), 1);
+    if ((entries && hipHostMalloc((void**)&sg.adj, entries * 4) != hipSuccess) ||
+        hipHostMalloc((void**)&sg.off, (rows + 1) * 4) != hipSuccess) {
       (void)hipGetLastError();
       return fail(SGC_ENOMEM, "pinned host staging of %llu bytes failed",
-                  (unsigned long long)(stage_entries * 4 + (stage_rows + 1) * 4));
+                  (unsigned long long)(entries * 4 + (rows + 1) * 4));
     }
     SGC_HIP(hipEventCreateWithFlags(&sg.free, hipEventDisableTiming));
   }
+  if (hipHostMalloc((void**)&R.h_iflag, std::max<uint64_t>(I, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SGC_ENOMEM, "pinned host flag staging of %llu bytes failed", (unsigned long long)I);
+  }
+  SGC_HIP(hipEventCreateWithFlags(&R.back_i, hipEventDisableTiming));
   for (int i = 0; i < 2; ++i) {
-    if (hipHostMalloc((void**)&R.h_flag[i], sim_entries) != hipSuccess ||
+    if (hipHostMalloc((void**)&R.h_flag[i], std::max<uint64_t>(J, 1)) != hipSuccess ||
         hipHostMalloc((void**)&R.h_cnt[i], kCntSlots * 8) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(SGC_ENOMEM, "pinned host flag staging of %llu bytes failed", (unsigned long long)sim_entries);
+      return fail(SGC_ENOMEM, "pinned host flag staging of %llu bytes failed", (unsigned long long)J);
     }
     SGC_HIP(hipEventCreateWithFlags(&R.up_j[i], hipEventDisableTiming));
     SGC_HIP(hipEventCreateWithFlags(&R.comp[i], hipEventDisableTiming));
@@ -351,101 +386,123 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
   uint64_t totals[kCntSlots] = {0};
   double ms_sim = 0, ms_sim_large = 0;
 
-  // ---- similarity over chips of block pairs: uploads on R.h2d, kernels on
-  // the context stream, flag read-back on R.d2h, host merges in between
+  // ---- similarity over chips: each group of blocks alone, then with every
+  // later block.  Uploads on R.h2d, kernels on the context stream, flag
+  // read-back on R.d2h, host merges in between.
   const size_t P = blocks.size() - 1;
-  const SimChipArrays A = sim_arrays(R.chip, C, RC, nullptr);
-  auto block_m = [&](size_t b) { return g->row_ptr[blocks[b + 1]] - g->row_ptr[blocks[b]]; };
-  auto stage_block = [&](Staging& sg, size_t b) -> int {
+  const SimChipArrays A = sim_arrays(R.chip, I, J, RC, nullptr);
+  auto range_m = [&](size_t b0, size_t b1) { return g->row_ptr[blocks[b1]] - g->row_ptr[blocks[b0]]; };
+  // rows of blocks [b0, b1) into pinned staging (offsets only for a
+  // page-locked graph)
+  auto stage_rows = [&](Staging& sg, size_t b0, size_t b1) -> int {
     if (sg.pending) SGC_HIP(hipEventSynchronize(sg.free));
     sg.pending = false;
     const double tb = now_ms();
-    // a page-locked graph is sent as it lies; only the offsets are staged
-    ooc_copy_rows(g, blocks[b], blocks[b + 1], 0, g->pinned ? nullptr : sg.adj, sg.off);
+    ooc_copy_rows(g, blocks[b0], blocks[b1], 0, g->pinned ? nullptr : sg.adj, sg.off);
     R.ms_host += now_ms() - tb;
     return SGC_OK;
   };
-  // block b's rows to chip entries [pos, pos + m_b), its offsets (from its
-  // first entry) to off_dev; `done` is recorded on the upload stream
-  auto upload_block = [&](Staging& sg, size_t b, uint64_t pos, uint32_t* off_dev, hipEvent_t done) -> int {
-    const uint64_t m_b = block_m(b);
-    const uint32_t nr = (uint32_t)(blocks[b + 1] - blocks[b]);
-    const int32_t* from = g->pinned ? g->adj + g->row_ptr[blocks[b]] : sg.adj;
-    if (m_b) SGC_HIP(hipMemcpyAsync(A.adj + pos, from, m_b * 4, hipMemcpyHostToDevice, R.h2d));
+  // blocks [b0, b1) to chip entries [pos, pos + their entries), their
+  // offsets (from their first entry) to off_dev; `done` is recorded on the
+  // upload stream
+  auto upload_rows = [&](Staging& sg, size_t b0, size_t b1, uint64_t pos, uint32_t* off_dev, hipEvent_t done) -> int {
+    const uint64_t mb = range_m(b0, b1);
+    const uint32_t nr = (uint32_t)(blocks[b1] - blocks[b0]);
+    const int32_t* from = g->pinned ? g->adj + g->row_ptr[blocks[b0]] : sg.adj;
+    if (mb) SGC_HIP(hipMemcpyAsync(A.adj + pos, from, mb * 4, hipMemcpyHostToDevice, R.h2d));
     SGC_HIP(hipMemcpyAsync(off_dev, sg.off, ((size_t)nr + 1) * 4, hipMemcpyHostToDevice, R.h2d));
     SGC_HIP(hipEventRecord(sg.free, R.h2d));
     sg.pending = true;
     SGC_HIP(hipEventRecord(done, R.h2d));
-    R.bytes_h2d += (double)m_b * 4 + (double)(nr + 1) * 4;
+    R.bytes_h2d += (double)mb * 4 + (double)(nr + 1) * 4;
     return SGC_OK;
   };
   bool used[2] = {false, false};  // comp[b] / back[b] recorded
   ChipDesc prev{};
   int prev_b = 0;
   size_t chip_no = 0;
+  // a chip's counters and its row block's flags, once they are back
   auto merge = [&](const ChipDesc& c, int b) -> int {
     SGC_HIP(hipEventSynchronize(R.back[b]));
     const double tb = now_ms();
-    const uint8_t* hf = R.h_flag[b];
-    if (c.j_first) {
-      ooc_merge_rows(g, c.r0j, c.r1j, hf, R.flag);
-      ooc_merge_rows(g, c.r0i, c.r1i, hf + c.mj, R.flag);
-    } else {
-      ooc_merge_rows(g, c.r0i, c.r1i, hf, R.flag);
-      if (c.r0j != c.r0i) ooc_merge_rows(g, c.r0j, c.r1j, hf + c.mi, R.flag);
-    }
+    if (c.mj) ooc_merge_rows(g, c.r0j, c.r1j, R.h_flag[b], R.flag);
     for (int q = 0; q < kCntSlots; ++q) totals[q] += R.h_cnt[b][q];
     R.ms_host += now_ms() - tb;
     ms_sim += elapsed(R.t0[b], R.t1[b]);
     ms_sim_large += elapsed(R.t0[b], R.tm[b]);
     return SGC_OK;
   };
-  for (size_t i = 0; g->m && i < P; ++i) {
+  // the last group's flags, once they are back
+  bool group_back = false;  // back_i recorded and its flags not merged yet
+  uint64_t back_r0 = 0, back_r1 = 0;
+  auto merge_group = [&]() -> int {
+    if (!group_back) return SGC_OK;
+    SGC_HIP(hipEventSynchronize(R.back_i));
+    const double tb = now_ms();
+    ooc_merge_rows(g, back_r0, back_r1, R.h_iflag, R.flag);
+    R.ms_host += now_ms() - tb;
+    group_back = false;
+    return SGC_OK;
+  };
+  const size_t total_chips = [&]() {
+    size_t t = 0;
+    for (size_t gi = 0; gi < P; gi += group) t += 1 + (P - std::min<size_t>(gi + group, P));
+    return t;
+  }();
+  bool any_group = false;
+  for (size_t gi = 0; g->m && gi < P; gi += group) {
+    const size_t ge = std::min<size_t>(gi + group, P);
     ChipDesc c;
-    c.r0i = blocks[i], c.r1i = blocks[i + 1];
-    c.mi = block_m(i);
+    c.r0i = blocks[gi], c.r1i = blocks[ge];
+    c.mi = range_m(gi, ge);
     const uint32_t nri = (uint32_t)(c.r1i - c.r0i);
-    // block i goes to chip entries [C, C + mi) once every chip of the
-    // previous block i has run
-    if ((rc = stage_block(R.stage[0], i))) return rc;
+    // the group goes to chip entries [J, J + mi) once every chip of the
+    // previous group has run
+    if ((rc = stage_rows(R.stage[0], gi, ge))) return rc;
     for (int b = 0; b < 2; ++b)
       if (used[b]) SGC_HIP(hipStreamWaitEvent(R.h2d, R.comp[b], 0));
-    if ((rc = upload_block(R.stage[0], i, C, R.off_i, R.up_i))) return rc;
+    if ((rc = upload_rows(R.stage[0], gi, ge, J, R.off_i, R.up_i))) return rc;
     SGC_HIP(hipStreamWaitEvent(s, R.up_i, 0));
-    if (nri) k_fill_src_rows<<<grid_for(nri, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0i, nri, R.off_i, A.src + C);
-    for (size_t j = i; j < P; ++j) {
+    if (nri) k_fill_src_rows<<<grid_for(nri, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0i, nri, R.off_i, A.src + J);
+    // the group's flags are rewritten once the previous group's are back
+    if (any_group) SGC_HIP(hipStreamWaitEvent(s, R.back_i, 0));
+    any_group = true;
+    for (size_t j = ge - 1; j < P; ++j) {
+      const bool alone = j + 1 == ge;  // the group's own chip
       const int b = (int)(chip_no & 1);
-      uint64_t base = C;  // the chip's first entry in adj / src
+      uint64_t base = J;   // the chip's first entry in adj / src / flag
+      uint64_t jpos = 0;   // its row block's first entry
       c.j_first = false;
       uint32_t nrj = 0;
-      if (j > i) {
+      if (!alone) {
         c.r0j = blocks[j], c.r1j = blocks[j + 1];
-        c.mj = block_m(j);
+        c.mj = range_m(j, j + 1);
         nrj = (uint32_t)(c.r1j - c.r0j);
-        // block j right before block i (buffer 0) or right after it (buffer
-        // 1), once the chip before last, which used this buffer, has run
-        const uint64_t pos = b == 0 ? C - c.mj : C + c.mi;
+        // block j right before the group (buffer 0) or right after it
+        // (buffer 1), once the chip before last, which used this buffer, has
+        // run
+        jpos = b == 0 ? J - c.mj : J + c.mi;
         Staging& sj = R.stage[1 + b];
-        if ((rc = stage_block(sj, j))) return rc;
+        if ((rc = stage_rows(sj, j, j + 1))) return rc;
         if (used[b]) SGC_HIP(hipStreamWaitEvent(R.h2d, R.comp[b], 0));
-        if ((rc = upload_block(sj, j, pos, R.off_j[b], R.up_j[b]))) return rc;
+        if ((rc = upload_rows(sj, j, j + 1, jpos, R.off_j[b], R.up_j[b]))) return rc;
         SGC_HIP(hipStreamWaitEvent(s, R.up_j[b], 0));
         if (nrj)
           k_fill_src_rows<<<grid_for(nrj, kBlock / kWave), kBlock, 0, s>>>((uint32_t)c.r0j, nrj, R.off_j[b],
-                                                                           A.src + pos);
+                                                                           A.src + jpos);
         if (b == 0) {
-          base = pos;
+          base = jpos;
           c.j_first = true;
         }
       } else {
         c.r0j = c.r0i, c.r1j = c.r1i, c.mj = 0;
       }
-      // flag array b is free once the chip before last has been read back
+      // buffer b's flags are free once the chip before last has been read back
       if (used[b]) SGC_HIP(hipStreamWaitEvent(s, R.back[b], 0));
-      // rows of the two blocks in this chip's layout; rows outside them are
-      // never read (the prep kernel reads degrees of entries with one end in
-      // each block only, every later kernel rows of the chip's owners and
-      // of their edges' short rows)
+      // rows of the group and the block in this chip's layout; rows outside
+      // them are never read (the prep kernel reads degrees of entries with
+      // one end in each only, every later kernel rows of the chip's owners
+      // and of their edges' short rows)
       if (nri)
         k_rows_range<<<grid_for(nri, kBlock), kBlock, 0, s>>>((uint32_t)c.r0i, nri, c.j_first ? (uint32_t)c.mj : 0u,
                                                               R.off_i, R.rb, R.re);
@@ -464,8 +521,9 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       view.pair.lo0 = (uint32_t)c.r0i, view.pair.hi0 = (uint32_t)c.r1i;
       view.pair.lo1 = (uint32_t)c.r0j, view.pair.hi1 = (uint32_t)c.r1j;
       view.pair.on = true;
+      view.pair.keep0 = !alone;  // the group's other entries were decided by its earlier chips
       PassBuffers cb = pb;
-      cb.flag = A.flag[b];
+      cb.flag = A.flag + base;
       cb.uedge = A.uedge;
       cb.units = A.units;
       cb.ucap = (uint32_t)(RC + 2);
@@ -479,24 +537,34 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       SGC_HIP(hipMemcpyAsync(R.h_cnt[b], pb.cnt, kCntSlots * 8, hipMemcpyDeviceToHost, s));
       SGC_HIP(hipEventRecord(R.comp[b], s));
       SGC_HIP(hipStreamWaitEvent(R.d2h, R.comp[b], 0));
-      if (m_chip) SGC_HIP(hipMemcpyAsync(R.h_flag[b], A.flag[b], m_chip, hipMemcpyDeviceToHost, R.d2h));
+      if (c.mj) SGC_HIP(hipMemcpyAsync(R.h_flag[b], A.flag + jpos, c.mj, hipMemcpyDeviceToHost, R.d2h));
       SGC_HIP(hipEventRecord(R.back[b], R.d2h));
       used[b] = true;
-      R.bytes_d2h += (double)m_chip;
-      // while the device works: merge the previous chip
+      R.bytes_d2h += (double)c.mj;
+      // while the device works: merge the previous chip, and the previous
+      // group once this group's own chip is queued
       if (chip_no > 0 && (rc = merge(prev, prev_b))) return rc;
+      if (alone && (rc = merge_group())) return rc;
       prev = c;
       prev_b = b;
       ++chip_no;
-      if (chip_no % 32 == 0) progress("ooc: %zu of %zu similarity chips issued", chip_no, P * (P + 1) / 2);
+      if (chip_no % 32 == 0) progress("ooc: %zu of %zu similarity chips issued", chip_no, total_chips);
     }
+    // the group's flags back to the host after its last chip (the d2h stream
+    // already waits for that chip)
+    if (c.mi) SGC_HIP(hipMemcpyAsync(R.h_iflag, A.flag + J, c.mi, hipMemcpyDeviceToHost, R.d2h));
+    SGC_HIP(hipEventRecord(R.back_i, R.d2h));
+    R.bytes_d2h += (double)c.mi;
+    group_back = true;
+    back_r0 = c.r0i, back_r1 = c.r1i;
   }
   if (chip_no && (rc = merge(prev, prev_b))) return rc;
+  if ((rc = merge_group())) return rc;
   SGC_HIP(hipStreamSynchronize(R.h2d));
   SGC_HIP(hipStreamSynchronize(s));
   if ((rc = check_unit_overflow(totals, sim_entries))) return rc;
   const size_t nsim = chip_no;
-  progress("ooc: similarity done over %zu chips of %zu blocks", nsim, P);
+  progress("ooc: similarity done over %zu chips of %zu blocks in groups of %u", nsim, P, group);
 
   // ---- core check, union-find, border, classify over row chips
   const double t_phase = now_ms();
@@ -629,6 +697,8 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
     oi->bytes_d2h = R.bytes_d2h;
     oi->ms_host = R.ms_host;
     oi->ms_phases = ms_phases;
+    oi->row_blocks = (uint32_t)(blocks.size() - 1);
+    oi->blocks_per_group = group;
   }
   return SGC_OK;
 }
@@ -644,9 +714,9 @@ extern "C" int sgc_ooc_min_budget(const sgc_host_graph* g, double eps, uint64_t*
   if (g->n >= (1ull << 31)) return fail(SGC_ERANGE, "n=%llu: vertex ids must fit int32", (unsigned long long)g->n);
   uint64_t maxd = 1;
   for (uint64_t x = 0; x < g->n; ++x) maxd = std::max<uint64_t>(maxd, g->row_ptr[x + 1] - g->row_ptr[x]);
-  // a chip of two blocks that each hold the longest row
-  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 + sim_chip_bytes(maxd, record_capacity(maxd, maxd)) +
-           offsets_bytes(maxd) + 4096;
+  // a chip of two single blocks that each hold the longest row
+  *bytes = resident_layout((uint32_t)g->n).bytes + 8192 +
+           sim_chip_bytes(maxd, maxd, record_capacity(maxd + 1, maxd)) + offsets_bytes(maxd, maxd) + 4096;
   return SGC_OK;
 }
 

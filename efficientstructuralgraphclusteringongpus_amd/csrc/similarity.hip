@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(uint32_t m, const uint32_t* __r
             if (count_all && f <= kFlagY)  // every edge to its owner, for exact counts
               f = (dv[j] > du[j] || (dv[j] == du[j] && x > row)) ? kFlagUUp : kFlagUDown;
           }
-          flag[k] = f;
+          if (need[j] || !(pf.keep0 && row - pf.lo0 < pf.hi0 - pf.lo0)) flag[k] = f;
           owned = f == kFlagUDown;
           yn = f == kFlagY ? 0x10000u : (f == kFlagN ? 1u : 0u);
         }

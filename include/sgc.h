@@ -287,21 +287,24 @@ int sgc_copy_bench(sgc_ctx* ctx, uint64_t bytes, double* ms);
 /* Out-of-core run details. */
 typedef struct {
   uint64_t device_bytes;     /* device memory the run allocated (<= the budget) */
-  uint32_t similarity_chips; /* chips (pairs of row blocks) of the similarity phase */
+  uint32_t similarity_chips; /* chips of the similarity phase (a group of row blocks, alone or with one later block) */
   uint32_t row_chips;        /* row chips of the core / cluster / border / classify passes */
   double bytes_h2d;          /* host-to-device bytes */
   double bytes_d2h;          /* device-to-host bytes */
   double ms_host;            /* host time planning, assembling chips and merging flags */
   double ms_phases;          /* wall time of the passes after similarity */
+  uint32_t row_blocks;       /* row blocks of the similarity phase */
+  uint32_t blocks_per_group; /* consecutive row blocks per group (4 where the budget allows, else 2 or 1) */
 } sgc_ooc_stats;
 
 /* Replaces the out-of-core GPUScan of GPUSCAN++O (SCANoffload.cu:922-1464,
  * chips GPUSCAN++O/Graph.cpp:414-576) for a host-resident graph of any size
  * (m may exceed 2^32; n < 2^31): at most `device_budget` bytes of device
- * memory are allocated.  Rows are cut into blocks of at most half a chip; the
- * similarity phase runs over every pair of blocks (each undecided edge in the
- * chip holding both of its rows), the later phases over chips of rows, with
- * the per-vertex state resident throughout.  Outputs as sgc_scan (host arrays,
+ * memory are allocated.  Rows are cut into blocks and consecutive blocks into
+ * groups; the similarity phase runs over each group alone and over each group
+ * with every later block (each undecided edge in the chip holding both of its
+ * rows), the later phases over chips of rows, with the per-vertex state
+ * resident throughout.  Outputs as sgc_scan (host arrays,
  * each optional) and bit-identical to it.  stats->ms_similarity fields are
  * device time summed over chips; the per-phase fields are filled when all
  * rows fit one chip; ms_total is wall time.  SGC_ENOMEM if the budget is

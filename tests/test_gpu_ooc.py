@@ -152,6 +152,37 @@ def test_ooc_hub_heavy_planted(ctx):
     assert_same(r, ref)
 
 
+def group_chips(blocks, k):
+    """Similarity chips of `blocks` row blocks in groups of k: each group
+    alone, then with every later block."""
+    return sum(1 + blocks - min(first + k, blocks) for first in range(0, blocks, k))
+
+
+def test_ooc_groups_of_blocks(ctx):
+    """Where the budget allows, four consecutive row blocks form a group that
+    stays on the device (flags included) while every later block passes
+    through: the group's earlier chips' flags must survive its later chips."""
+    g = sgc.HostGraph.rmat(16, 16, seed=9)
+    for eps, mu in ((0.3, 3), (0.6, 2)):
+        ref = resident(ctx, g, eps, mu)
+        r, info = sgc.scan_ooc(ctx, g, eps, mu, budget_for(g, eps, g.m // 3), want_sim=True)
+        say("groups", eps, info.row_blocks, info.blocks_per_group, info.similarity_chips)
+        assert info.blocks_per_group == 4 and info.row_blocks > 4
+        assert info.similarity_chips == group_chips(info.row_blocks, 4)
+        assert_same(r, ref)
+
+
+def test_ooc_single_blocks_at_the_minimum_budget(ctx):
+    """At the smallest budget a chip holds two blocks of the longest row, so
+    blocks are not grouped: every pair of blocks is one chip."""
+    g = sgc.HostGraph.rmat(13, 8, seed=10)
+    r, info = sgc.scan_ooc(ctx, g, 0.4, 2, sgc.ooc_min_budget(g, 0.4), want_sim=True)
+    say("single blocks", info.row_blocks, info.blocks_per_group, info.similarity_chips)
+    assert info.blocks_per_group == 1 and info.row_blocks > 2
+    assert info.similarity_chips == group_chips(info.row_blocks, 1)
+    assert_same(r, resident(ctx, g, 0.4, 2))
+
+
 def test_ooc_rejects_budgets_below_the_minimum(ctx):
     g = sgc.HostGraph.rmat(12, 8, seed=1)
     for budget in (1024, sgc.ooc_min_budget(g, 0.5) - 64 * 1024):
