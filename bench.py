@@ -395,6 +395,7 @@ def ooc_bench(sgc, ctx, args):
                                f"({'page-locked' if pin_s is not None else 'pageable'}), "
                                f"device budget {budget} bytes", "n": g.n, "edges": E},
         "ooc": {"device_bytes": info.device_bytes, "similarity_chips": info.similarity_chips,
+                "row_blocks": info.row_blocks, "blocks_per_group": info.blocks_per_group,
                 "row_chips": info.row_chips, "bytes_h2d": info.bytes_h2d, "bytes_d2h": info.bytes_d2h,
                 "ms_host": round(info.ms_host, 1), "ms_phases": round(info.ms_phases, 1),
                 "similarity_device_ms": round(r.stats.ms_similarity, 1),

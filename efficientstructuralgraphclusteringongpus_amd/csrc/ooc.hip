@@ -50,6 +50,22 @@ __global__ void __launch_bounds__(kBlock) k_rows_range(uint32_t r0, uint32_t nr,
   }
 }
 
+// Per similarity chip: the counters, and the planner counts of the chip's
+// rows [a0, a1) and [b0, b1) (the only rows its prep, planner and similarity
+// kernels touch; the later phases clear every row again).
+__global__ void __launch_bounds__(kBlock) k_chip_init(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1,
+                                                      uint32_t* __restrict__ cnt_u, uint64_t* __restrict__ work_u,
+                                                      uint64_t* __restrict__ cnt) {
+  const uint64_t na = a1 - a0, total = na + (b1 - b0);
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += stride) {
+    const uint64_t x = i < na ? a0 + i : b0 + (i - na);
+    cnt_u[x] = 0;
+    work_u[x] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kCntSlots) cnt[threadIdx.x] = 0;
+}
+
 // src[k] = r0 + i for the entries [off[i], off[i+1]) of chip row i; one wave
 // per row.
 __global__ void __launch_bounds__(kBlock) k_fill_src_rows(uint32_t r0, uint32_t nr, const uint32_t* __restrict__ off,
@@ -527,7 +543,11 @@ int run_ooc(OocRun& R, double eps, int32_t mu, uint64_t budget, int32_t* labels,
       cb.uedge = A.uedge;
       cb.units = A.units;
       cb.ucap = (uint32_t)(RC + 2);
-      launch_init(&view, cb, s);
+      {
+        const uint32_t b0 = alone ? 0u : (uint32_t)c.r0j, b1 = alone ? 0u : (uint32_t)c.r1j;
+        k_chip_init<<<grid_for((uint64_t)nri + (b1 - b0), kBlock), kBlock, 0, s>>>(
+            (uint32_t)c.r0i, (uint32_t)c.r1i, b0, b1, cb.cnt_u, cb.work_u, cb.cnt);
+      }
       launch_prep_packed(&view, cb, p, s);
       SGC_HIP(hipEventRecord(R.t0[b], s));
       launch_similarity_packed(&view, cb, p, ctx->cu_count, ctx->sim_impl, R.tm[b], s, (uint32_t)W, rounds);

@@ -254,7 +254,7 @@ __device__ __forceinline__ uint32_t edges_per_unit(uint32_t cnt, uint64_t work, 
 // (GPUSCAN++/GPUScan.cu:140-169).  redo: the second round of a pruned pass
 // (owned entries counted in cnt[kCntRedoOwned]; nothing to do when
 // cnt[kCntRedo] is 0).
-__global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __restrict__ rb,
+__global__ void __launch_bounds__(kBlock) k_plan(uint32_t x0, uint32_t x1, const uint32_t* __restrict__ rb,
                                                  const uint32_t* __restrict__ re, SimTuning tune,
                                                  const uint32_t* __restrict__ cnt_u,
                                                  const uint64_t* __restrict__ work_u,
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
   }
   if (vstate) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += stride) {
+    for (uint64_t x = x0 + (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < x1; x += stride) {
       const uint32_t d = re[x] - rb[x], y = ny[x], no = nn[x];
       vstate[x] = y >= (uint32_t)mu ? kVCore : (d - no < (uint32_t)mu ? kVNonCore : kVUndecided);
     }
@@ -288,13 +288,13 @@ __global__ void __launch_bounds__(kBlock) k_plan(uint32_t n, const uint32_t* __r
   unsigned long long* const ctr[3] = {(unsigned long long*)&cnt[kCntUnits1], (unsigned long long*)&cnt[kCntUnits2],
                                       (unsigned long long*)&cnt[kCntTiny]};
   const uint64_t tile = (uint64_t)kBlock * V;
-  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {  // block-uniform
+  for (uint64_t base = x0 + (uint64_t)blockIdx.x * tile; base < x1; base += (uint64_t)gridDim.x * tile) {  // block-uniform
     // records of owner x: [rs, rs + owned) in uedge, its row [start, stop) in adj
     uint32_t owned[V], c1[V], c2[V], c3[V], per[V], start[V], stop[V], rs[V];
 #pragma unroll
     for (uint32_t j = 0; j < V; ++j) {
       const uint64_t x = base + j * kBlock + threadIdx.x;
-      owned[j] = x < n ? cnt_u[x] : 0u;
+      owned[j] = x < x1 ? cnt_u[x] : 0u;
       if (rec_start && owned[j] && rec_start[x] - r0 >= w) owned[j] = 0;  // another round's owner
     }
     uint32_t need[3] = {0, 0, 0};
@@ -1257,7 +1257,7 @@ void launch_plan(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStr
   if (g->m == 0) return;
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
-  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(0, g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
                                                    b.ucap, b.tiny, b.cnt, p.prune ? b.core : nullptr, b.ny, b.nn,
                                                    p.mu, false);
 }
@@ -1304,6 +1304,28 @@ __global__ void k_round_reset(uint64_t* __restrict__ cnt) {
     cnt[kCntTiny] = 0;
   }
 }
+// The rows of a chip (its pair filter's one or two row ranges), or all rows
+// of a whole graph: the only rows whose planner counts a pass touches.
+struct RowRanges {
+  uint32_t lo[2], hi[2];
+  int count;
+};
+RowRanges chip_rows(const sgc_dgraph* g) {
+  if (!g->pair.on) return RowRanges{{0, 0}, {g->n, 0}, 1};
+  if (g->pair.lo1 == g->pair.lo0 && g->pair.hi1 == g->pair.hi0)
+    return RowRanges{{g->pair.lo0, 0}, {g->pair.hi0, 0}, 1};
+  return RowRanges{{g->pair.lo0, g->pair.lo1}, {g->pair.hi0, g->pair.hi1}, 2};
+}
+
+// A second scanned range continues the first: out[i] += the first range's
+// last start + its last count.
+__global__ void __launch_bounds__(kBlock) k_scan_carry(uint32_t* __restrict__ out, uint32_t count,
+                                                       const uint32_t* __restrict__ last_start,
+                                                       const uint32_t* __restrict__ last_count) {
+  const uint32_t add = *last_start + *last_count;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) out[i] += add;
+}
 }  // namespace
 
 void launch_prep_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, hipStream_t s) {
@@ -1311,9 +1333,19 @@ void launch_prep_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p,
   k_prep<false, false, kRecCount><<<grid_for(g->m, kBlock), kBlock, 0, s>>>(
       g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror, p.eps, false, g->pair, b.flag, b.cnt_u,
       b.work_u, b.uedge, b.cnt, nullptr, nullptr);
-  size_t tmp_bytes = b.scan_tmp_bytes;
-  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tmp_bytes, b.cnt_u, b.rec_start, g->n, s);
-  (void)hipMemcpyAsync(b.rec_cur, b.rec_start, (size_t)g->n * 4, hipMemcpyDeviceToDevice, s);
+  // each owner's first record: an exclusive scan of the owned counts over
+  // the chip's rows, the second range continuing where the first ends
+  const RowRanges rr = chip_rows(g);
+  for (int q = 0; q < rr.count; ++q) {
+    const uint32_t x0 = rr.lo[q], count = rr.hi[q] - rr.lo[q];
+    if (!count) continue;
+    size_t tmp_bytes = b.scan_tmp_bytes;
+    (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tmp_bytes, b.cnt_u + x0, b.rec_start + x0, count, s);
+    if (q == 1 && rr.hi[0] > rr.lo[0])
+      k_scan_carry<<<grid_for(count, kBlock), kBlock, 0, s>>>(b.rec_start + x0, count, b.rec_start + rr.hi[0] - 1,
+                                                              b.cnt_u + rr.hi[0] - 1);
+    (void)hipMemcpyAsync(b.rec_cur + x0, b.rec_start + x0, (size_t)count * 4, hipMemcpyDeviceToDevice, s);
+  }
 }
 
 void launch_similarity_packed(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu_count, int impl,
@@ -1327,15 +1359,18 @@ void launch_similarity_packed(const sgc_dgraph* g, const PassBuffers& b, ScanPar
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
+  const RowRanges rr = chip_rows(g);
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t r0 = r * w;
     k_round_reset<<<1, kWave, 0, s>>>(b.cnt);
     k_prep<false, false, kRecCompact><<<grid_for(g->m, kBlock), kBlock, 0, s>>>(
         g->m, g->row_begin, g->row_end, g->adj, g->src, g->ndeg, g->mirror, p.eps, false, g->pair, b.flag, b.cnt_u,
         b.work_u, b.uedge, b.cnt, nullptr, nullptr, b.rec_cur, b.rec_start, r0, w);
-    k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
-                                                     b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, false,
-                                                     b.rec_start, r0, w);
+    for (int q = 0; q < rr.count; ++q)
+      if (rr.hi[q] > rr.lo[q])
+        k_plan<<<grid_for(rr.hi[q] - rr.lo[q], kBlock), kBlock, 0, s>>>(
+            rr.lo[q], rr.hi[q], g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units, b.ucap, b.tiny, b.cnt, nullptr,
+            nullptr, nullptr, p.mu, false, b.rec_start, r0, w);
     k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,
                                                                nullptr, g->woff, window_count(g->n) + 1, b.uedge, b.units,
                                                                b.ucap, b.cnt, kCntUnits2, kCntQueue2,
@@ -1389,7 +1424,7 @@ void launch_redo(const sgc_dgraph* g, const PassBuffers& b, ScanParams p, int cu
   SimTuning t = g->tuning;
   if (t.small_max_degree > kT1MaxDeg) t.small_max_degree = kT1MaxDeg;
   t.tiny_work = 0;
-  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
+  k_plan<<<grid_for(g->n, kBlock), kBlock, 0, s>>>(0, g->n, g->row_begin, g->row_end, t, b.cnt_u, b.work_u, b.units,
                                                    b.ucap, b.tiny, b.cnt, nullptr, nullptr, nullptr, p.mu, true);
   const uint32_t cus = (uint32_t)(cu_count > 0 ? cu_count : 256);
   k_sim_units<LargeTier><<<cus, LargeTier::kThreads, 0, s>>>(g->n, g->m, g->src, g->adj, g->mirror, p.eps, b.flag,

@@ -65,7 +65,8 @@ def test_rmat28_ef32_out_of_core_24gib(ctx):
     t0 = time.perf_counter()
     r, info = sgc.scan_ooc(ctx, g, eps, mu, budget, want_sim=True)
     wall = time.perf_counter() - t0
-    say(f"out of core: {wall:.1f} s ({g.m // 2 / wall / 1e9:.3f} GTEPS), {info.similarity_chips} similarity chips, "
+    say(f"out of core: {wall:.1f} s ({g.m // 2 / wall / 1e9:.3f} GTEPS), {info.similarity_chips} similarity chips "
+        f"({info.row_blocks} row blocks in groups of {info.blocks_per_group}), "
         f"{info.row_chips} row chips, {info.bytes_h2d / 1e9:.0f} GB to the device, {info.bytes_d2h / 1e9:.0f} GB back, "
         f"device {info.device_bytes / 2**30:.2f} GiB, similarity device time {r.stats.ms_similarity / 1e3:.1f} s, "
         f"host work {info.ms_host / 1e3:.1f} s, later phases {info.ms_phases / 1e3:.1f} s; cores {r.stats.cores}, "
