@@ -10,9 +10,9 @@ uniform sample of 2M entries against the oracle's per-edge similarity; labels an
 flags against the oracle's phases after similarity over the run's own flag
 array; the degree-only pre-decision count against the oracle's.  The
 resident path indexes entries with 32 bits and cannot hold this graph, so
-the same pass is run again within a 96 GiB budget (about 9 blocks instead of
-81, so every edge is decided in a different chip): labels, core flags and
-every similarity flag must be identical.
+the same pass is run again within a 96 GiB budget (24 row blocks in groups
+of four instead of 123, so edges are decided in chips of other rows):
+labels, core flags and every similarity flag must be identical.
 
 It runs only with SGC_C5=1: generating the graph alone takes minutes, beyond
 what the default GPU suite may spend.
