@@ -18,9 +18,12 @@
 //    not similar (the reference's predicate in its exact threshold form).
 //    The result is written to both directed entries.
 //    Small tier: owners of degree <= 1024, 256-thread workgroups, the row as
-//    a linear-probing hash table.  Large tier: 1024-thread workgroups, the
-//    row as a cuckoo hash (whole, or in runs of 2^20-id windows) or one
-//    window at a time as an exact 128 KB bitmap.
+//    a two-table cuckoo hash (linear probing if an insertion chain fails).
+//    Large tier: 1024-thread workgroups, the row as a cuckoo hash (whole, or
+//    in runs of 2^20-id windows) or one window at a time as an exact 128 KB
+//    bitmap.
+// 4. k_sim_tiny: owners whose edges' whole merge work is small, one edge per
+//    lane, merging the two rows with the same threshold and early exit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
