@@ -31,8 +31,9 @@ def resident(ctx, g, eps, mu):
 
 
 def budget_for(g, eps, chip_entries):
-    """The smallest accepted budget plus room for `chip_entries` more
-    similarity-chip block entries (44 bytes each)."""
+    """The smallest accepted budget plus 44 bytes per `chip_entries`: about
+    that many more entries per block for chips of two single blocks (43
+    bytes per block entry), fewer per block once blocks are grouped."""
     return sgc.ooc_min_budget(g, eps) + 44 * int(chip_entries)
 
 
