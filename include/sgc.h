@@ -361,12 +361,13 @@ int sgc_server_set_graph_launch(sgc_server* srv, int enable);
 int sgc_server_set_batching(sgc_server* srv, uint64_t cutoff_entries, uint32_t max_queries);
 int sgc_server_get_batching(const sgc_server* srv, uint64_t* cutoff_entries, uint32_t* max_queries);
 /* Picks the batching cut-off from measurement: for every admitted graph, the
- * device time of one (eps, mu) query replayed in a slot against the device
- * time per query of a batch of its slots (the faster of two runs after a
- * warm-up each); the cut-off becomes the graph size (in entries) that
- * minimises the summed time over all graphs when the graphs up to it are
- * batched and the others replayed, 0 if batching none is best.  Needs an
- * idle server (SGC_EBUSY otherwise). */
+ * wall time per query of one (eps, mu) query on each of its slots (up to the
+ * batch size), replayed at once on the slots' streams, against one batch of
+ * the same queries (the faster of two runs after a warm-up each); the
+ * cut-off becomes the graph size (in entries) that minimises the summed time
+ * over all graphs when the graphs up to it are batched and the others
+ * replayed, 0 if batching none is best.  Needs an idle server (SGC_EBUSY
+ * otherwise). */
 int sgc_server_autotune_batching(sgc_server* srv, double eps, int32_t mu, uint64_t* cutoff_entries);
 /* Admission: uploads g and computes every edge's common-neighbour count once
  * (precompute_ms: that pass).  SGC_ENOMEM if the budget cannot hold it; the
