@@ -18,7 +18,7 @@ launch come from the committed rocprofv3 counter summary of this workload
 each of the 12 configurations, scaled by the same counters of a plain copy),
 summed over the launches actually timed and divided by their hipEvent-timed
 durations on the library's stream: achieved GB/s, and frac against the
-8 TB/s peak.  The algorithmic-byte model of the survey (sum over the
+8 TB/s peak.  The algorithmic-byte model (sum over the
 kernel's undecided edges of 4 * (s + min(l, s * ceil(log2(l + 1))))) is
 reported beside it under alg_model; early exit and LDS-resident owner rows
 move far fewer bytes than that model, so its ratio to the peak passes 1.
