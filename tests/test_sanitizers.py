@@ -79,7 +79,7 @@ def test_thread_sanitizer_clean(scratch):
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "0 failed checks" in r.stdout
-    # the planted race is reported: the tool is live in this build
+    # the planted race is reported: the tool is live in this instrumented binary
     p = run(exe, ["race"], TSAN_ENV, scratch)
     assert "ThreadSanitizer: data race" in p.stderr and p.returncode == 66
 
@@ -96,6 +96,6 @@ def test_address_and_undefined_sanitizers_clean(scratch):
     assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "0 failed checks" in r.stdout
-    # the planted out-of-bounds read is reported: the tool is live in this build
+    # the planted out-of-bounds read is reported: the tool is live in this instrumented binary
     p = run(exe, ["overflow"], {}, scratch)
     assert "heap-buffer-overflow" in p.stderr and p.returncode != 0
