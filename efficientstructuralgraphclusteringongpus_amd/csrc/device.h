@@ -71,6 +71,10 @@ struct sgc_ctx {
   hipEvent_t ev[8] = {};
   int sim_impl = sgc::kSimTiered;
   void* scratch = nullptr;  // this GPU's static scratch (sgc_scan_device_noalloc)
+  // page-locked staging for graph uploads from pageable memory (two
+  // buffers, allocated at the first such upload and kept until destroy)
+  void* bounce[2] = {nullptr, nullptr};
+  hipEvent_t bounce_done[2] = {nullptr, nullptr};
   bool prune = false;       // core-check pruning in passes that return no similarity flags (sgc_ctx_set_pruning)
 };
 

@@ -168,6 +168,10 @@ extern "C" void sgc_ctx_destroy(sgc_ctx* ctx) {
   if (ctx->work.ptr) (void)hipFree(ctx->work.ptr);
   if (ctx->counters.ptr) (void)hipFree(ctx->counters.ptr);
   if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->bounce[i]) (void)hipHostFree(ctx->bounce[i]);
+    if (ctx->bounce_done[i]) (void)hipEventDestroy(ctx->bounce_done[i]);
+  }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -209,6 +213,52 @@ extern "C" int sgc_ctx_set_pruning(sgc_ctx* ctx, int enable) {
   return SGC_OK;
 }
 
+namespace {
+constexpr size_t kBounceBytes = 64u << 20;
+
+// Host-to-device copy on the context stream.  A pageable source goes through
+// the context's two page-locked buffers: the next chunk is copied in while
+// the previous one is sent (the runtime's own staging of a pageable copy
+// runs about a third slower).  The buffers are free again on return.
+hipError_t upload_bytes(sgc_ctx* ctx, void* dst, const void* src, size_t bytes, bool src_pinned) {
+  if (!bytes) return hipSuccess;
+  if (!src_pinned && !ctx->bounce[0]) {
+    bool ok = true;
+    for (int i = 0; i < 2 && ok; ++i)
+      ok = hipHostMalloc(&ctx->bounce[i], kBounceBytes) == hipSuccess &&
+           hipEventCreateWithFlags(&ctx->bounce_done[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {  // the runtime stages the copy instead
+      (void)hipGetLastError();
+      for (int i = 0; i < 2; ++i) {
+        if (ctx->bounce[i]) (void)hipHostFree(ctx->bounce[i]);
+        if (ctx->bounce_done[i]) (void)hipEventDestroy(ctx->bounce_done[i]);
+        ctx->bounce[i] = nullptr;
+        ctx->bounce_done[i] = nullptr;
+      }
+    }
+  }
+  if (src_pinned || !ctx->bounce[0]) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  bool used[2] = {false, false};
+  hipError_t e = hipSuccess;
+  for (size_t off = 0, i = 0; off < bytes && e == hipSuccess; off += kBounceBytes, ++i) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(kBounceBytes, bytes - off);
+    if (used[b]) e = hipEventSynchronize(ctx->bounce_done[b]);  // its last chunk is on the device
+    if (e != hipSuccess) break;
+    parallel_copy(ctx->bounce[b], (const char*)src + off, len);
+    e = hipMemcpyAsync((char*)dst + off, ctx->bounce[b], len, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->bounce_done[b], ctx->stream);
+    used[b] = true;
+  }
+  for (int b = 0; b < 2; ++b)
+    if (used[b]) {
+      const hipError_t w = hipEventSynchronize(ctx->bounce_done[b]);
+      if (e == hipSuccess) e = w;
+    }
+  return e;
+}
+}  // namespace
+
 extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out) {
   clear_error();
   if (!ctx || !g || !out) return fail(SGC_EINVAL, "null argument");
@@ -223,10 +273,7 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   d->m = (uint32_t)g->m;
   std::vector<uint32_t> rp32(g->n + 1);
   uint64_t maxd = 0;
-  for (uint64_t u = 0; u <= g->n; ++u) {
-    rp32[u] = (uint32_t)g->row_ptr[u];
-    if (u < g->n) maxd = std::max<uint64_t>(maxd, g->row_ptr[u + 1] - g->row_ptr[u]);
-  }
+  narrow_row_ptr(g, rp32.data(), &maxd);
   d->max_degree = (uint32_t)maxd;
   auto cleanup = [d]() {
     if (d->mirror) (void)hipFree(d->mirror);
@@ -271,9 +318,8 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
       d->woff = nullptr;  // the kernels find window boundaries while streaming instead
     }
   }
-  hipError_t e = hipMemcpyAsync(d->row_ptr, rp32.data(), (g->n + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess && g->m)
-    e = hipMemcpyAsync(d->adj, g->adj, g->m * 4, hipMemcpyHostToDevice, ctx->stream);
+  hipError_t e = upload_bytes(ctx, d->row_ptr, rp32.data(), (g->n + 1) * 4, false);
+  if (e == hipSuccess && g->m) e = upload_bytes(ctx, d->adj, g->adj, g->m * 4, g->pinned);
   if (e == hipSuccess) {
     launch_fill_src(d, ctx->stream);
     launch_mirror(d, ctx->stream);

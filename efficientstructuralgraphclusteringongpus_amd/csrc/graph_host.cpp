@@ -99,6 +99,28 @@ static inline uint64_t lower_bound_u64(const int32_t* a, uint64_t lo, uint64_t h
   return lo;
 }
 
+void narrow_row_ptr(const sgc_host_graph* g, uint32_t* out, uint64_t* max_degree) {
+  const uint64_t* rp = g->row_ptr;
+  const int64_t n = (int64_t)g->n;
+  uint64_t maxd = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxd)
+  for (int64_t u = 0; u <= n; ++u) {
+    out[u] = (uint32_t)rp[u];
+    if (u < n) maxd = std::max<uint64_t>(maxd, rp[u + 1] - rp[u]);
+  }
+  *max_degree = maxd;
+}
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const size_t piece = 1 << 20;
+  const int64_t pieces = (int64_t)((bytes + piece - 1) / piece);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < pieces; ++p) {
+    const size_t off = (size_t)p * piece;
+    memcpy((char*)dst + off, (const char*)src + off, std::min(piece, bytes - off));
+  }
+}
+
 int validate_graph(const sgc_host_graph* g, bool check_symmetry) {
   const uint64_t n = g->n;
   if (g->row_ptr[0] != 0 || g->row_ptr[n] != g->m)

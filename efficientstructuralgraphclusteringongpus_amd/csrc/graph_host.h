@@ -26,5 +26,10 @@ int validate_graph(const sgc_host_graph* g, bool check_symmetry);
 // set to null, as soon as they are no longer read.
 int build_from_pairs(uint64_t n, uint64_t count, const uint32_t* src, const uint32_t* dst,
                      sgc_host_graph** out, uint32_t** release = nullptr);
+// g's row offsets narrowed to uint32 into out[0..n] (the caller guarantees
+// m < 2^32) and its largest degree, over OpenMP threads.
+void narrow_row_ptr(const sgc_host_graph* g, uint32_t* out, uint64_t* max_degree);
+// memcpy over OpenMP threads, for large copies into page-locked staging.
+void parallel_copy(void* dst, const void* src, size_t bytes);
 
 }  // namespace sgc
