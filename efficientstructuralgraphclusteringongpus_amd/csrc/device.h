@@ -198,8 +198,13 @@ int check_unit_overflow(const uint64_t* cnt_host, uint64_t m);
 // Kernel launchers (kernels.hip, similarity.hip).  All run on `s` and never
 // synchronise.
 void launch_fill_src(const sgc_dgraph* g, hipStream_t s);
-// fills g->mirror (after launch_fill_src); needs symmetric input
+// fills g->mirror (after launch_fill_src) by a search per entry; entries
+// without a reverse entry get 0xFFFFFFFF
 void launch_mirror(const sgc_dgraph* g, hipStream_t s);
+// fills g->mirror (after launch_fill_src) by a stable sort of the entries by
+// neighbour, or by launch_mirror where the input is not symmetric or the
+// sort's memory is not there; synchronises the stream
+hipError_t build_mirror(const sgc_dgraph* g, hipStream_t s);
 // fills g->ndeg
 void launch_neighbor_degrees(const sgc_dgraph* g, hipStream_t s);
 // 2^20-id windows of the large-tier bitmap covering n ids, and the kernel

@@ -322,7 +322,9 @@ extern "C" int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgra
   if (e == hipSuccess && g->m) e = upload_bytes(ctx, d->adj, g->adj, g->m * 4, g->pinned);
   if (e == hipSuccess) {
     launch_fill_src(d, ctx->stream);
-    launch_mirror(d, ctx->stream);
+    e = build_mirror(d, ctx->stream);
+  }
+  if (e == hipSuccess) {
     launch_neighbor_degrees(d, ctx->stream);
     launch_window_offsets(d, ctx->stream);
     e = hipGetLastError();

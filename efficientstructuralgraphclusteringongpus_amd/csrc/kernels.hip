@@ -5,6 +5,8 @@
 // consecutive lanes) so each row issues at most one atomic per wave.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include "device.h"
 #include "device_util.h"
 #include "scan_phases.h"
@@ -111,6 +113,41 @@ __global__ void __launch_bounds__(kBlock) k_mirror(uint32_t m, const uint32_t* _
       if (found) mirror[p] = k;
     }
   }
+}
+
+// The reverse-entry index by sorting.  In a symmetric CSR with sorted rows,
+// the entries stably sorted by neighbour id come in the order of their
+// reverse entries: row v lists its neighbours u ascending, and the entries
+// (u, v) ordered by v, then by position (so by u), are ordered the same way.
+// So the entry (u, v) at k that the sort puts at j has its reverse at j
+// exactly when entry j is (v, u), and then entry j's reverse is k: the index
+// is written at j, in order, not scattered.  Each pair is checked, and an
+// input where some is not falls back to the search (k_mirror).  The sort's
+// values carry the entry's row beside its position, so the check reads no
+// other row.
+__global__ void __launch_bounds__(kBlock) k_rows_and_positions(uint32_t m, const uint32_t* __restrict__ src,
+                                                               uint64_t* __restrict__ vals) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += stride)
+    vals[k] = ((uint64_t)src[k] << 32) | k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mirror_from_sorted(uint32_t m, const uint32_t* __restrict__ src,
+                                                               const int32_t* __restrict__ adj,
+                                                               const uint32_t* __restrict__ keys,
+                                                               const uint64_t* __restrict__ vals,
+                                                               uint32_t* __restrict__ mirror,
+                                                               unsigned long long* __restrict__ bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  unsigned long long miss = 0;
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < m; j += stride) {
+    const uint64_t val = vals[j];
+    const uint32_t k = (uint32_t)val, u = (uint32_t)(val >> 32), v = keys[j];
+    const bool ok = src[j] == v && (uint32_t)adj[j] == u;
+    mirror[j] = ok ? k : 0xFFFFFFFFu;
+    miss += ok ? 0u : 1u;
+  }
+  block_atomic_add(bad, miss);
 }
 
 // ndeg[k] = d(adj[k]), so the prep kernel streams the neighbour degrees
@@ -482,6 +519,46 @@ void launch_mirror(const sgc_dgraph* g, hipStream_t s) {
   // words of entries without a reverse entry are never written by k_mirror
   (void)hipMemsetAsync(g->mirror, 0xFF, (size_t)g->m * 4, s);
   k_mirror<<<grid_for(g->m, kBlock), kBlock, 0, s>>>(g->m, g->row_ptr, g->src, g->adj, g->mirror);
+}
+
+hipError_t build_mirror(const sgc_dgraph* g, hipStream_t s) {
+  if (!g->m || !g->mirror) return hipSuccess;
+  const uint32_t m = g->m;
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < (uint64_t)g->n) ++bits;
+  size_t tmp_bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint64_t*)nullptr, (uint64_t*)nullptr, m, 0, bits, s);
+  // keys out, values in and out, the sort's scratch and the mismatch count
+  char* mem = nullptr;
+  const size_t o_keys = 0, o_vin = ((size_t)m * 4 + 255) / 256 * 256, o_vout = o_vin + (size_t)m * 8,
+               o_tmp = o_vout + (size_t)m * 8, o_bad = o_tmp + (tmp_bytes + 255) / 256 * 256;
+  if (hipMalloc(&mem, o_bad + 256) != hipSuccess) {
+    (void)hipGetLastError();
+    launch_mirror(g, s);  // not enough room to sort: search each reverse entry
+    return hipGetLastError();
+  }
+  uint32_t* keys = (uint32_t*)(mem + o_keys);
+  uint64_t* vin = (uint64_t*)(mem + o_vin);
+  uint64_t* vout = (uint64_t*)(mem + o_vout);
+  unsigned long long* bad = (unsigned long long*)(mem + o_bad);
+  k_rows_and_positions<<<grid_for(m, kBlock), kBlock, 0, s>>>(m, g->src, vin);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(mem + o_tmp, tmp_bytes, (const uint32_t*)g->adj, keys, vin, vout,
+                                                     m, 0, bits, s);
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, 8, s);
+  if (e == hipSuccess) {
+    k_mirror_from_sorted<<<grid_for(m, kBlock), kBlock, 0, s>>>(m, g->src, g->adj, keys, vout, g->mirror, bad);
+    e = hipGetLastError();
+  }
+  unsigned long long h_bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h_bad, bad, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(mem);
+  if (e == hipSuccess && h_bad) {  // some entry lacks its reverse: the search finds those that exist
+    launch_mirror(g, s);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 void launch_init(const sgc_dgraph* g, const PassBuffers& b, hipStream_t s) {
