@@ -275,6 +275,27 @@ def test_repeat_runs_identical(ctx):
     assert np.array_equal(a.labels, b.labels) and np.array_equal(a.sim, b.sim)
 
 
+def test_pageable_and_page_locked_uploads_agree(ctx):
+    """The upload stages a pageable graph through the context's page-locked
+    buffers in 64 MB chunks (R-MAT-20 at edge factor 16 takes two) and sends
+    a page-locked one as it lies: the same results either way, in a second
+    context too (its own staging), and the same as the oracle on a sample."""
+    say("pageable vs page-locked upload")
+    g = sgc.HostGraph.rmat(20, 16, seed=3)
+    assert g.m * 4 > 64 << 20
+    a = run_gpu(ctx, g, 0.4, 4)
+    with sgc.Context(0) as ctx2:
+        b = run_gpu(ctx2, g, 0.4, 4)
+    g.pin()
+    c = run_gpu(ctx, g, 0.4, 4)
+    for r in (b, c):
+        assert np.array_equal(a.sim, r.sim) and np.array_equal(a.labels, r.labels)
+    rp, adj = np.array(g.row_ptr), np.array(g.adj)
+    sample = np.random.default_rng(3).choice(len(adj), size=200_000, replace=False).astype(np.uint64)
+    _, ref = ob.orc_similarity_entries(rp, adj, 0.4, sample)
+    assert np.array_equal(a.sim[sample.astype(np.int64)], ref)
+
+
 def test_rmat20_sampled_oracle_and_label_rules(ctx):
     """Full-size consistency: similarity of a random 2% of entries against the
     oracle, and labels re-derived from the GPU's own flags by the vectorised
