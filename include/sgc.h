@@ -171,7 +171,14 @@ int sgc_ctx_set_pruning(sgc_ctx* ctx, int enable);
 typedef struct sgc_dgraph sgc_dgraph;
 
 /* Copies a host CSR into HBM (uint32 offsets; requires m < 2^32, else
- * SGC_ERANGE - use sgc_scan_ooc for larger graphs). */
+ * SGC_ERANGE - use sgc_scan_ooc for larger graphs) and builds the device
+ * indexes (entry rows, reverse entries, neighbour degrees, window offsets).
+ * A graph not in page-locked memory (sgc_graph_pin) is staged through two
+ * 64 MB page-locked buffers that the context allocates at its first such
+ * upload and keeps until sgc_ctx_destroy.  Building the reverse-entry index
+ * sorts the entries in about 20 bytes of temporary device memory per entry;
+ * without that room, or for an input whose entries lack their reverse, the
+ * index is built by a search instead.  Synchronises the context stream. */
 int sgc_dgraph_upload(sgc_ctx* ctx, const sgc_host_graph* g, sgc_dgraph** out);
 void sgc_dgraph_free(sgc_dgraph* dg);
 
